@@ -1,0 +1,316 @@
+"""ctypes view of a shared library that exports the acd-alm C interface
+(alm.h, approx/approx.h, spmv/spmv.h, thread_pool/thread_pool.h).
+
+The same class drives this package's GPU library (lib/libacdalm.so) and,
+in tests, the reference compiled from its own sources; it is also the
+binding a Python embedder would write against the original library
+(see INTEGRATION.md).  Arrays are numpy float64/uint32, copied in and out.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import tempfile
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PACKAGE_ROOT = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PACKAGE_ROOT, "lib", "libacdalm.so")
+
+vp = C.c_void_p
+dp = C.POINTER(C.c_double)
+u32p = C.POINTER(C.c_uint32)
+sz = C.c_size_t
+
+_libc = C.CDLL(None)
+_libc.fopen.restype = vp
+_libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+_libc.fclose.argtypes = [vp]
+_libc.fflush.argtypes = [vp]
+
+
+def _d(a):
+    return a.ctypes.data_as(dp) if a is not None else None
+
+
+def _u(a):
+    return a.ctypes.data_as(u32p)
+
+
+def f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.shape[0] != n:
+        raise ValueError("length mismatch")
+    return a
+
+
+class LogCapture:
+    """A C FILE* writing to a temporary file; .text after close()."""
+
+    def __init__(self, enabled=True):
+        self.enabled = enabled
+        self.fp = None
+        self.text = ""
+        if enabled:
+            fd, self.path = tempfile.mkstemp(prefix="acdlog")
+            os.close(fd)
+            self.fp = _libc.fopen(self.path.encode(), b"w")
+
+    def close(self):
+        if self.fp:
+            _libc.fflush(self.fp)
+            _libc.fclose(self.fp)
+            self.fp = None
+            with open(self.path) as f:
+                self.text = f.read()
+            os.unlink(self.path)
+        return self.text
+
+
+class CLib:
+    """Bindings for a library exporting the original interface."""
+
+    def __init__(self, path: str = LIB_PATH, *, gpu_extras: bool = None):
+        mode = getattr(os, "RTLD_LOCAL", 0) | getattr(os, "RTLD_NOW", 2)
+        self.path = path
+        self.lib = C.CDLL(path, mode=mode)
+        L = self.lib
+        sig = {
+            "sparse_matrix_init": (None, []),
+            "sparse_matrix_make": (vp, [sz, sz, sz, u32p, u32p, dp, C.c_int]),
+            "sparse_matrix_copy": (vp, [vp, C.c_int]),
+            "sparse_matrix_free": (C.c_int, [vp]),
+            "sparse_matrix_ncolumns": (sz, [vp]),
+            "sparse_matrix_nrows": (sz, [vp]),
+            "sparse_matrix_nnz": (sz, [vp]),
+            "sparse_matrix_rows": (u32p, [vp]),
+            "sparse_matrix_columns": (u32p, [vp]),
+            "sparse_matrix_values": (dp, [vp]),
+            "sparse_matrix_row_permute": (C.c_int, [vp, dp, sz, dp, C.c_int]),
+            "sparse_matrix_col_permute": (C.c_int, [vp, dp, sz, dp, C.c_int]),
+            "sparse_matrix_multiply": (C.c_int, [dp, sz, vp, dp, sz, C.c_int, vp]),
+            "sparse_matrix_multiply_2": (C.c_int, [C.POINTER(dp), sz, vp,
+                                                    C.POINTER(dp), sz, C.c_int, vp]),
+            "sparse_matrix_read": (vp, [vp, C.c_int]),
+            "approx_make": (vp, [vp, sz, dp, dp, sz, dp, dp, dp]),
+            "approx_free": (C.c_int, [vp]),
+            "approx_matrix": (vp, [vp]),
+            "approx_nrhs": (sz, [vp]),
+            "approx_nvars": (sz, [vp]),
+            "approx_rhs": (dp, [vp]),
+            "approx_weight": (dp, [vp]),
+            "approx_linear": (dp, [vp]),
+            "approx_lower": (dp, [vp]),
+            "approx_upper": (dp, [vp]),
+            "approx_update": (C.c_int, [vp]),
+            "approx_solve": (C.c_int, [dp, sz, vp, sz, C.c_double, C.c_double,
+                                       C.c_double, vp, sz, dp, C.c_double, vp]),
+            "mpg_solve": (C.c_int, [dp, sz, vp, sz, C.c_double, C.c_double,
+                                    C.c_double, vp, sz, dp, C.c_double, vp]),
+            "cd_solve": (C.c_int, [dp, sz, vp, sz, C.c_double, C.c_double,
+                                   C.c_double, vp, sz, dp, C.c_double, vp]),
+            "alm_make": (vp, [vp, sz, dp, sz, dp, dp, dp, dp, dp]),
+            "alm_free": (C.c_int, [vp]),
+            "alm_matrix": (vp, [vp]),
+            "alm_nrhs": (sz, [vp]),
+            "alm_nvars": (sz, [vp]),
+            "alm_rhs": (dp, [vp]),
+            "alm_linear": (dp, [vp]),
+            "alm_lower": (dp, [vp]),
+            "alm_upper": (dp, [vp]),
+            "alm_lambda_lower": (dp, [vp]),
+            "alm_lambda_upper": (dp, [vp]),
+            "alm_solve": (C.c_int, [vp, sz, dp, sz, dp, sz, vp, dp, vp]),
+            "alm_read": (vp, [vp]),
+            "read_doubles": (None, [vp, dp, sz]),
+            "thread_pool_init": (vp, [C.c_uint]),
+            "thread_pool_free": (None, [vp]),
+            "thread_pool_count": (sz, [vp]),
+        }
+        self.missing = []
+        for name, (res, args) in sig.items():
+            try:
+                fn = getattr(L, name)
+            except AttributeError:
+                self.missing.append(name)
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        if gpu_extras is None:
+            gpu_extras = hasattr(L, "acd_set_device")
+        self.gpu = gpu_extras
+        if gpu_extras:
+            self._bind_gpu()
+        L.sparse_matrix_init()
+
+    def _bind_gpu(self):
+        L = self.lib
+        for name, res, args in [
+            ("acd_set_device", C.c_int, [C.c_int]),
+            ("acd_last_error", C.c_int, [C.c_char_p, sz]),
+            ("acd_synchronize", C.c_int, []),
+            ("acd_device_alloc", vp, [sz]),
+            ("acd_device_free", None, [vp]),
+            ("acd_copy_to_device", C.c_int, [vp, vp, sz]),
+            ("acd_copy_to_host", C.c_int, [vp, vp, sz]),
+            ("acd_spmv_device", C.c_int, [vp, C.c_int, vp, vp]),
+            ("acd_spmv_device_time", C.c_int, [vp, C.c_int, vp, vp, C.c_int,
+                                               C.POINTER(C.c_double)]),
+            ("acd_matrix_stored_entries", C.c_uint64, [vp, C.c_int]),
+        ]:
+            fn = getattr(L, name, None)
+            if fn is None:
+                self.missing.append(name)
+                continue
+            fn.restype = res
+            fn.argtypes = args
+
+    # ---- matrices ----
+    def make_matrix(self, lp, permute=0, rows=None, cols=None, vals=None,
+                    nrows=None, ncols=None):
+        rows = np.ascontiguousarray(lp.rows if rows is None else rows, np.uint32)
+        cols = np.ascontiguousarray(lp.cols if cols is None else cols, np.uint32)
+        vals = f64(lp.vals if vals is None else vals)
+        nrows = lp.nrows if nrows is None else nrows
+        ncols = lp.ncols if ncols is None else ncols
+        m = self.lib.sparse_matrix_make(ncols, nrows, vals.shape[0], _u(rows),
+                                        _u(cols), _d(vals), permute)
+        if not m:
+            raise MemoryError("sparse_matrix_make returned NULL")
+        return m
+
+    def copy_matrix(self, m, permute):
+        return self.lib.sparse_matrix_copy(m, permute)
+
+    def free_matrix(self, m):
+        return self.lib.sparse_matrix_free(m)
+
+    def coo(self, m):
+        L = self.lib
+        nnz = L.sparse_matrix_nnz(m)
+        if nnz == 0:
+            return (np.zeros(0, np.uint32), np.zeros(0, np.uint32),
+                    np.zeros(0))
+        r = np.ctypeslib.as_array(L.sparse_matrix_rows(m), (nnz,)).copy()
+        c = np.ctypeslib.as_array(L.sparse_matrix_columns(m), (nnz,)).copy()
+        v = np.ctypeslib.as_array(L.sparse_matrix_values(m), (nnz,)).copy()
+        return r, c, v
+
+    def shape(self, m):
+        return (self.lib.sparse_matrix_nrows(m),
+                self.lib.sparse_matrix_ncolumns(m))
+
+    def permute(self, m, src, row, direction):
+        src = f64(src)
+        dest = np.zeros_like(src)
+        fn = (self.lib.sparse_matrix_row_permute if row
+              else self.lib.sparse_matrix_col_permute)
+        rc = fn(m, _d(dest), src.shape[0], _d(src), direction)
+        return rc, dest
+
+    def multiply(self, m, x, transpose=0, pool=None):
+        nr, nc = self.shape(m)
+        ny, nx = (nc, nr) if transpose else (nr, nc)
+        x = f64(x, nx)
+        y = np.zeros(ny)
+        self.lib.sparse_matrix_multiply(_d(y), ny, m, _d(x), nx, transpose,
+                                        pool)
+        return y
+
+    def multiply_2(self, m, x0, x1, transpose=0, pool=None):
+        nr, nc = self.shape(m)
+        ny, nx = (nc, nr) if transpose else (nr, nc)
+        x0, x1 = f64(x0, nx), f64(x1, nx)
+        y0, y1 = np.zeros(ny), np.zeros(ny)
+        ys = (dp * 2)(_d(y0), _d(y1))
+        xs = (dp * 2)(_d(x0), _d(x1))
+        self.lib.sparse_matrix_multiply_2(ys, ny, m, xs, nx, transpose, pool)
+        return y0, y1
+
+    # ---- APPROX ----
+    def approx_make(self, m, lp, weight=None, linear="lp"):
+        lin = lp.linear if isinstance(linear, str) else linear
+        a = self.lib.approx_make(
+            m, lp.nrows, _d(f64(lp.rhs)),
+            _d(f64(weight)) if weight is not None else None, lp.ncols,
+            _d(f64(lin)) if lin is not None else None, _d(f64(lp.lower)),
+            _d(f64(lp.upper)))
+        if not a:
+            raise MemoryError("approx_make returned NULL")
+        return a
+
+    def approx_free(self, a):
+        return self.lib.approx_free(a)
+
+    def approx_update(self, a):
+        return self.lib.approx_update(a)
+
+    def _inner(self, fn, a, x0, niter, max_pg, max_value, min_delta, log,
+               period, offset, pool):
+        x = f64(x0).copy()
+        diag = np.full(5, np.nan)
+        cap = LogCapture(log)
+        rc = fn(_d(x), x.shape[0], a, niter, max_pg, max_value, min_delta,
+                cap.fp, period, _d(diag), offset, pool)
+        return rc, x, diag, cap.close()
+
+    def approx_solve(self, a, x0, niter, max_pg=0.0, max_value=-np.inf,
+                     min_delta=0.0, log=False, period=0, offset=0.0,
+                     pool=None):
+        return self._inner(self.lib.approx_solve, a, x0, niter, max_pg,
+                           max_value, min_delta, log, period, offset, pool)
+
+    def mpg_solve(self, a, x0, niter, max_pg=0.0, max_value=-np.inf,
+                  min_delta=0.0, log=False, period=0, offset=0.0, pool=None):
+        return self._inner(self.lib.mpg_solve, a, x0, niter, max_pg,
+                           max_value, min_delta, log, period, offset, pool)
+
+    def cd_solve(self, a, x0, niter, max_pg=0.0, max_value=-np.inf,
+                 min_delta=0.0, log=False, period=0, offset=0.0, pool=None):
+        return self._inner(self.lib.cd_solve, a, x0, niter, max_pg,
+                           max_value, min_delta, log, period, offset, pool)
+
+    # ---- ALM ----
+    def alm_make(self, m, lp):
+        a = self.lib.alm_make(m, lp.nrows, _d(f64(lp.rhs)), lp.ncols,
+                              _d(f64(lp.linear)), _d(f64(lp.lower)),
+                              _d(f64(lp.upper)), _d(f64(lp.lambda_lower)),
+                              _d(f64(lp.lambda_upper)))
+        if not a:
+            raise MemoryError("alm_make returned NULL")
+        return a
+
+    def alm_free(self, a):
+        return self.lib.alm_free(a)
+
+    def alm_solve(self, a, niter, x0, lam0, log=False, pool=None):
+        x = f64(x0).copy()
+        lam = f64(lam0).copy()
+        diag = np.full(3, np.nan)
+        cap = LogCapture(log)
+        rc = self.lib.alm_solve(a, niter, _d(x), x.shape[0], _d(lam),
+                                lam.shape[0], cap.fp, _d(diag), pool)
+        return rc, x, lam, diag, cap.close()
+
+    # ---- GPU extras ----
+    def last_error(self):
+        buf = C.create_string_buffer(256)
+        code = self.lib.acd_last_error(buf, 256)
+        return code, buf.value.decode(errors="replace")
+
+
+_LIB = None
+
+
+def load(path: str = LIB_PATH) -> CLib:
+    """The GPU library, loaded once per process.  Raises if it is not
+    built: there is no fallback."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(path):
+            raise RuntimeError(
+                f"{path} is not built; run __graft_entry__.build() or "
+                "make -C acd-alm_amd")
+        _LIB = CLib(path)
+    return _LIB

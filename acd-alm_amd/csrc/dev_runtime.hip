@@ -1,0 +1,147 @@
+// dev_runtime.hip -- HIP runtime plumbing behind acd_dev.h: device
+// selection, the library stream, zero-filled allocations, copies and
+// error capture.  Replaces huge_alloc/huge_alloc.default.inc:6-32 of
+// pkhuong/acd-alm (64-B aligned zeroed allocation) with hipMalloc, whose
+// allocations are 256-B aligned, plus an explicit zero fill.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "acd_dev.h"
+
+namespace {
+int g_device = 0;
+int g_inited = 0;
+hipStream_t g_stream = nullptr;
+int g_cus = 0;
+int g_err = 0;
+char g_errtext[256];
+
+int note(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    if (g_err == 0) {
+      g_err = (int)e;
+      snprintf(g_errtext, sizeof g_errtext, "%s: %s", what,
+               hipGetErrorString(e));
+    }
+    return (int)e;
+  }
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int acd_dev_set_device(int device) {
+  if (g_inited && device != g_device) return -1;
+  g_device = device;
+  return 0;
+}
+
+int acd_dev_device(void) { return g_device; }
+
+int acd_dev_init(void) {
+  if (g_inited) return 0;
+  int n = 0;
+  if (note(hipGetDeviceCount(&n), "hipGetDeviceCount")) return g_err;
+  if (n <= 0 || g_device >= n) {
+    acd_dev_record_error(-2, "no HIP device visible");
+    return -2;
+  }
+  if (note(hipSetDevice(g_device), "hipSetDevice")) return g_err;
+  if (note(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking),
+           "hipStreamCreate"))
+    return g_err;
+  hipDeviceProp_t prop;
+  if (note(hipGetDeviceProperties(&prop, g_device), "hipGetDeviceProperties"))
+    return g_err;
+  g_cus = prop.multiProcessorCount;
+  g_inited = 1;
+  return 0;
+}
+
+int acd_dev_cu_count(void) { return g_cus; }
+
+void* acd_dev_stream(void) { return (void*)g_stream; }
+
+int acd_dev_set_stream(void* s) {
+  g_stream = (hipStream_t)s;
+  return 0;
+}
+
+void* acd_dev_malloc(size_t bytes) {
+  if (acd_dev_init()) return nullptr;
+  if (bytes == 0) bytes = 256;
+  void* p = nullptr;
+  if (note(hipMalloc(&p, bytes), "hipMalloc")) return nullptr;
+  if (note(hipMemsetAsync(p, 0, bytes, g_stream), "hipMemsetAsync")) {
+    hipFree(p);
+    return nullptr;
+  }
+  return p;
+}
+
+void acd_dev_free(void* p) {
+  if (p == nullptr) return;
+  // Frees are ordered after pending work on the library stream.
+  hipStreamSynchronize(g_stream);
+  note(hipFree(p), "hipFree");
+}
+
+int acd_dev_h2d(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return 0;
+  return note(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream),
+              "hipMemcpy H2D") ||
+         note(hipStreamSynchronize(g_stream), "sync after H2D");
+}
+
+int acd_dev_d2h(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return 0;
+  return note(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream),
+              "hipMemcpy D2H") ||
+         note(hipStreamSynchronize(g_stream), "sync after D2H");
+}
+
+int acd_dev_d2d(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return 0;
+  return note(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice,
+                             g_stream),
+              "hipMemcpy D2D");
+}
+
+int acd_dev_memset(void* dst, int byte, size_t bytes) {
+  if (bytes == 0) return 0;
+  return note(hipMemsetAsync(dst, byte, bytes, g_stream), "hipMemsetAsync");
+}
+
+int acd_dev_sync(void) {
+  return note(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
+}
+
+int acd_dev_last_error(char* buf, size_t len) {
+  int e = g_err;
+  if (buf && len) {
+    strncpy(buf, e ? g_errtext : "", len - 1);
+    buf[len - 1] = 0;
+  }
+  g_err = 0;
+  g_errtext[0] = 0;
+  return e;
+}
+
+void acd_dev_record_error(int code, const char* what) {
+  if (g_err == 0) {
+    g_err = code;
+    snprintf(g_errtext, sizeof g_errtext, "%s", what);
+  }
+}
+
+void acd_dev_check(int code, const char* what) {
+  if (code == 0) return;
+  char buf[256];
+  acd_dev_last_error(buf, sizeof buf);
+  fprintf(stderr, "acd-alm-amd: %s failed (%d): %s\n", what, code, buf);
+  abort();
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+/* acd_gpu.h -- additions of the MI355X library that have no counterpart
+ * in pkhuong/acd-alm.  Everything declared in alm.h, approx/approx.h and
+ * spmv/spmv.h keeps its original meaning; nothing here is required to use
+ * them.
+ */
+#ifndef ACD_GPU_H
+#define ACD_GPU_H
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+#include "spmv/spmv.h"
+#include "approx/approx.h"
+#include "alm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- device, errors ---- */
+/* Choose the HIP device before sparse_matrix_init().  Returns 0, or -1
+ * once the library is already bound to another device. */
+int acd_set_device(int device);
+/* First HIP error recorded since the previous call (0 if none); its text
+ * is copied into buf (may be NULL). */
+int acd_last_error(char * buf, size_t len);
+/* Wait for all work queued by the library. */
+int acd_synchronize(void);
+
+/* ---- device-resident data ---- */
+void * acd_device_alloc(size_t bytes);          /* zero-filled */
+void acd_device_free(void * p);
+int acd_copy_to_device(void * dst, const void * src, size_t bytes);
+int acd_copy_to_host(void * dst, const void * src, size_t bytes);
+/* d_y <- op(A) d_x on device pointers, enqueued on the library stream. */
+int acd_spmv_device(const sparse_matrix_t *, int transpose,
+                    double * d_y, const double * d_x);
+/* Mean milliseconds per launch of the device product over reps
+ * back-to-back launches, timed with HIP events on the library stream. */
+int acd_spmv_device_time(const sparse_matrix_t *, int transpose,
+                         double * d_y, const double * d_x, int reps,
+                         double * OUT_ms_per_launch);
+/* Stored entries of the operator (padding to the slice length included). */
+uint64_t acd_matrix_stored_entries(const sparse_matrix_t *, int transpose);
+
+/* ---- solver options ---- */
+enum acd_precision {
+        ACD_FP64 = 0,    /* FP64 everywhere: parity form */
+        ACD_MIXED = 1    /* FP32 matrix values and x/z/u vectors, FP64 row
+                            accumulation, duals, norms and scalars */
+};
+
+struct acd_alm_options {
+        double stop_violation;   /* 1e-5  (alm.c:306) */
+        double stop_pg;          /* 1e-5  (alm.c:308) */
+        double precision_floor;  /* 1e-7  (alm.c:244) */
+        double min_delta;        /* 1e-11 (alm.c:256) */
+        int precision;           /* enum acd_precision */
+};
+void acd_alm_default_options(struct acd_alm_options *);
+/* alm_solve with explicit stopping constants and precision. */
+int alm_solve_ex(alm_t *, size_t niter, double * x, size_t nvars,
+                 double * lambda, size_t nconstraints, FILE * log,
+                 double * OUT_diagnosis, const struct acd_alm_options *);
+
+/* Counters of the most recent alm_solve / approx_solve / mpg_solve. */
+struct acd_solve_stats {
+        uint64_t inner_iterations;   /* APPROX / SPG iterations */
+        uint64_t outer_iterations;   /* ALM iterations */
+        uint64_t launches;           /* solver kernel launches */
+        double device_ms;            /* time inside solver launches */
+};
+void acd_last_solve_stats(struct acd_solve_stats *);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

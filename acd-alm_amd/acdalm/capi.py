@@ -30,6 +30,20 @@ _libc.fclose.argtypes = [vp]
 _libc.fflush.argtypes = [vp]
 
 
+class AlmOptions(C.Structure):
+    """struct acd_alm_options (include/acd_gpu.h)."""
+    _fields_ = [("stop_violation", C.c_double), ("stop_pg", C.c_double),
+                ("precision_floor", C.c_double), ("min_delta", C.c_double),
+                ("precision", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    """struct acd_solve_stats (include/acd_gpu.h)."""
+    _fields_ = [("inner_iterations", C.c_uint64),
+                ("outer_iterations", C.c_uint64), ("launches", C.c_uint64),
+                ("device_ms", C.c_double)]
+
+
 def _d(a):
     return a.ctypes.data_as(dp) if a is not None else None
 
@@ -158,6 +172,10 @@ class CLib:
             ("acd_spmv_device_time", C.c_int, [vp, C.c_int, vp, vp, C.c_int,
                                                C.POINTER(C.c_double)]),
             ("acd_matrix_stored_entries", C.c_uint64, [vp, C.c_int]),
+            ("acd_alm_default_options", None, [C.POINTER(AlmOptions)]),
+            ("alm_solve_ex", C.c_int, [vp, sz, dp, sz, dp, sz, vp, dp,
+                                       C.POINTER(AlmOptions)]),
+            ("acd_last_solve_stats", None, [C.POINTER(SolveStats)]),
         ]:
             fn = getattr(L, name, None)
             if fn is None:
@@ -292,6 +310,24 @@ class CLib:
         rc = self.lib.alm_solve(a, niter, _d(x), x.shape[0], _d(lam),
                                 lam.shape[0], cap.fp, _d(diag), pool)
         return rc, x, lam, diag, cap.close()
+
+    def alm_solve_ex(self, a, niter, x0, lam0, log=False, **opts):
+        o = AlmOptions()
+        self.lib.acd_alm_default_options(C.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        x = f64(x0).copy()
+        lam = f64(lam0).copy()
+        diag = np.full(3, np.nan)
+        cap = LogCapture(log)
+        rc = self.lib.alm_solve_ex(a, niter, _d(x), x.shape[0], _d(lam),
+                                   lam.shape[0], cap.fp, _d(diag), C.byref(o))
+        return rc, x, lam, diag, cap.close()
+
+    def stats(self):
+        st = SolveStats()
+        self.lib.acd_last_solve_stats(C.byref(st))
+        return st
 
     # ---- GPU extras ----
     def last_error(self):

@@ -93,6 +93,81 @@ int acd_dev_spmv_time(const struct acd_dev_matrix *, int transpose,
                       double * y, const double * x, int reps,
                       double * OUT_ms);
 
+/* ---- device-resident solver (dev_solver.hip) ---- */
+/* Original-order vectors of a device problem that the host uploads. */
+enum acd_vec {
+        ACD_V_RHS = 0,        /* b (approx rhs)               [m] */
+        ACD_V_WEIGHT,         /* row weights                  [m] */
+        ACD_V_LINEAR,         /* approx linear term c'        [n] */
+        ACD_V_LOWER,          /* x lower bounds               [n] */
+        ACD_V_UPPER,          /* x upper bounds               [n] */
+        ACD_V_ALM_LINEAR,     /* LP objective c               [n] */
+        ACD_V_LAM_LO,         /* multiplier bounds            [m] */
+        ACD_V_LAM_HI,         /*                              [m] */
+        ACD_V_X,              /* x in/out                     [n] */
+        ACD_V_LAMBDA,         /* multipliers in/out           [m] */
+        ACD_V_COUNT
+};
+
+enum acd_mode { ACD_MODE_APPROX = 1, ACD_MODE_MPG = 2, ACD_MODE_ALM = 3 };
+
+struct acd_solve_request {
+        int mode;                 /* enum acd_mode */
+        int log;                  /* emit log events */
+        uint64_t niter;           /* inner (APPROX/MPG) or outer (ALM) cap */
+        double max_pg, max_value, min_delta, offset;
+        uint64_t period;
+        /* ALM only */
+        double stop_violation, stop_pg, precision_floor, inner_min_delta;
+        int precision;            /* enum acd_precision */
+};
+
+struct acd_solve_result {
+        int ret;                  /* solver return code */
+        int error;                /* 0, or an ACD_ERR_* code */
+        double diag[5];
+        uint64_t inner_iterations, outer_iterations, launches;
+        double device_ms;
+};
+
+enum acd_err {
+        ACD_ERR_NONE = 0,
+        ACD_ERR_UNBOUNDED = 1,    /* max |zp| not finite (step_project.inc:90) */
+        ACD_ERR_ESTIMATE = 2,     /* expected improvement > 0 (approx.c:376) */
+        ACD_ERR_BARRIER = 3,      /* grid barrier timed out */
+        ACD_ERR_LAUNCH = 4
+};
+
+/* Log events, replayed by the host in order. */
+enum acd_event_type {
+        ACD_EV_APPROX_BEGIN = 1,  /* new inner solve: reset restart flag */
+        ACD_EV_RESTART,           /* a[0] = iteration */
+        ACD_EV_NEWLINE,           /* the stdout newline after restarts */
+        ACD_EV_PRINT,             /* k, value, ng, pg, step, delta */
+        ACD_EV_MPG,               /* k, pg, value */
+        ACD_EV_ALM_ITER,          /* k, max_viol, |r|, pg, value, c'x */
+        ACD_EV_ALM_TAIL           /* stalled, mean weight */
+};
+
+struct acd_event {
+        int32_t type, pad;
+        uint64_t k;
+        double a[6];
+};
+
+typedef void (*acd_event_sink)(void * ctx, const struct acd_event * ev);
+
+struct acd_dev_lp;
+struct acd_dev_lp * acd_dev_lp_create(const struct acd_dev_matrix * orig,
+                                      const struct acd_dev_matrix * perm);
+void acd_dev_lp_free(struct acd_dev_lp *);
+double * acd_dev_lp_vec(struct acd_dev_lp *, int which);
+/* Permute b, w, c', l, u into the working copy and recompute the step
+ * sizes there (approx/approx.c:188-216). */
+int acd_dev_approx_update(struct acd_dev_lp *);
+int acd_dev_solve(struct acd_dev_lp *, const struct acd_solve_request *,
+                  struct acd_solve_result *, acd_event_sink, void * ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -240,6 +240,7 @@ int build_sell(struct acd_sell* op, uint32_t nmajor, uint32_t nminor,
   AL(maxlen, 1);
   if (hipMemsetAsync(op->rowlen, 0, sizeof(uint32_t) * (nmajor ? nmajor : 1), S()) ||
       hipMemsetAsync(width, 0, sizeof(uint64_t) * (op->nslices + 1), S()) ||
+      hipMemsetAsync(op->off, 0, sizeof(uint64_t) * (op->nslices + 1), S()) ||
       hipMemsetAsync(maxlen, 0, sizeof(uint32_t), S())) {
     rc = -4;
     goto done;

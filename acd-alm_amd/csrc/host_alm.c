@@ -1,0 +1,185 @@
+/* host_alm.c -- the alm.h entry points.
+ *
+ * Mirrors alm.c:8-364 of pkhuong/acd-alm.  The host object keeps the
+ * caller-visible vectors; alm_solve uploads them, runs the whole outer
+ * loop on the device (penalised objective, APPROX inner solve, residual,
+ * multiplier / weight / precision updates, stopping test), and brings back
+ * x, lambda and the updated row weights.  The inner solver is APPROX, as
+ * alm.h:82-89 documents (the original's alm.c:255 calls cd_solve).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "acd_approx.h"
+
+static double * copy_default(const double * x, size_t n, double missing)
+{
+        double * out = malloc(sizeof(double) * (n ? n : 1));
+        if (out == NULL) return NULL;
+        for (size_t i = 0; i < n; i++) out[i] = x ? x[i] : missing;
+        return out;
+}
+
+/* alm.c:59-83 */
+alm_t * alm_make(sparse_matrix_t * constraints, size_t nrhs,
+                 const double * rhs, size_t nvars, const double * linear,
+                 const double * lower, const double * upper,
+                 const double * lambda_lower, const double * lambda_upper)
+{
+        alm_t * alm = calloc(1, sizeof(*alm));
+        if (alm == NULL) return NULL;
+        alm->nrhs = nrhs;
+        alm->nvars = nvars;
+        alm->matrix = constraints;
+        alm->linear = copy_default(linear, nvars, 0);
+        alm->approx = approx_make(constraints, nrhs, rhs, NULL, nvars, NULL,
+                                  lower, upper);
+        alm->lambda_lower = copy_default(lambda_lower, nrhs, -HUGE_VAL);
+        alm->lambda_upper = copy_default(lambda_upper, nrhs, HUGE_VAL);
+        if (!alm->linear || !alm->approx || !alm->lambda_lower
+            || !alm->lambda_upper) {
+                alm_free(alm);
+                return NULL;
+        }
+        return alm;
+}
+
+/* alm.c:85-96 */
+int alm_free(alm_t * alm)
+{
+        if (alm == NULL) return 0;
+        free(alm->linear);
+        free(alm->lambda_lower);
+        free(alm->lambda_upper);
+        approx_free(alm->approx);
+        memset(alm, 0, sizeof(*alm));
+        free(alm);
+        return 0;
+}
+
+/* alm.c:23-43 */
+sparse_matrix_t * alm_matrix(alm_t * alm) { return alm->matrix; }
+size_t alm_nrhs(alm_t * alm) { return alm->nrhs; }
+size_t alm_nvars(alm_t * alm) { return alm->nvars; }
+double * alm_linear(alm_t * alm) { return alm->linear; }
+double * alm_lambda_lower(alm_t * alm) { return alm->lambda_lower; }
+double * alm_lambda_upper(alm_t * alm) { return alm->lambda_upper; }
+double * alm_rhs(alm_t * alm) { return approx_rhs(alm->approx); }
+double * alm_lower(alm_t * alm) { return approx_lower(alm->approx); }
+double * alm_upper(alm_t * alm) { return approx_upper(alm->approx); }
+
+void acd_alm_default_options(struct acd_alm_options * o)
+{
+        o->stop_violation = 1e-5;
+        o->stop_pg = 1e-5;
+        o->precision_floor = 1e-7;
+        o->min_delta = 1e-11;
+        o->precision = ACD_FP64;
+}
+
+int alm_solve_ex(alm_t * alm, size_t niter, double * x, size_t nvars,
+                 double * lambda, size_t nconstraints, FILE * log,
+                 double * OUT_diagnosis, const struct acd_alm_options * opt)
+{
+        struct acd_alm_options o;
+        if (opt) o = *opt;
+        else acd_alm_default_options(&o);
+        ACD_REQUIRE(nvars == alm->nvars, "alm_solve: nvars mismatch");
+        ACD_REQUIRE(nconstraints == alm->nrhs, "alm_solve: nconstraints mismatch");
+        approx_t * a = alm->approx;
+        struct acd_dev_lp * dev = a->dev;
+        size_t m = alm->nrhs, n = alm->nvars;
+        if (acd_approx_upload(a)
+            || acd_dev_h2d(acd_dev_lp_vec(dev, ACD_V_ALM_LINEAR), alm->linear, n * 8)
+            || acd_dev_h2d(acd_dev_lp_vec(dev, ACD_V_LAM_LO), alm->lambda_lower, m * 8)
+            || acd_dev_h2d(acd_dev_lp_vec(dev, ACD_V_LAM_HI), alm->lambda_upper, m * 8)
+            || acd_dev_h2d(acd_dev_lp_vec(dev, ACD_V_X), x, n * 8)
+            || acd_dev_h2d(acd_dev_lp_vec(dev, ACD_V_LAMBDA), lambda, m * 8))
+                acd_fail("alm_solve: upload");
+        struct acd_solve_request q;
+        memset(&q, 0, sizeof q);
+        q.mode = ACD_MODE_ALM;
+        q.log = log != NULL;
+        q.niter = niter;
+        q.stop_violation = o.stop_violation;
+        q.stop_pg = o.stop_pg;
+        q.precision_floor = o.precision_floor;
+        q.inner_min_delta = o.min_delta;
+        q.precision = o.precision;
+        struct acd_solve_result r;
+        struct acd_log_ctx ctx = {log, 0};
+        int rc = acd_dev_solve(dev, &q, &r, acd_log_sink, &ctx);
+        acd_check_result(&r, rc, "alm_solve");
+        acd_record_stats(&r);
+        /* x, lambda, and the state the original leaves in its approx
+         * object: the updated row weights and the last penalised linear
+         * term (alm.c:106-119, :194-212). */
+        if (acd_dev_d2h(x, acd_dev_lp_vec(dev, ACD_V_X), n * 8)
+            || acd_dev_d2h(lambda, acd_dev_lp_vec(dev, ACD_V_LAMBDA), m * 8)
+            || acd_dev_d2h(a->weight, acd_dev_lp_vec(dev, ACD_V_WEIGHT), m * 8)
+            || (niter && acd_dev_d2h(a->linear,
+                                     acd_dev_lp_vec(dev, ACD_V_LINEAR), n * 8)))
+                acd_fail("alm_solve: download");
+        if (OUT_diagnosis != NULL)
+                for (int i = 0; i < 3; i++) OUT_diagnosis[i] = r.diag[i];
+        if (log) fflush(log);
+        return r.ret;
+}
+
+/* alm.c:289-322 */
+int alm_solve(alm_t * alm, size_t niter, double * x, size_t nvars,
+              double * lambda, size_t nconstraints, FILE * log,
+              double * OUT_diagnosis, thread_pool_t * pool)
+{
+        (void)pool;
+        return alm_solve_ex(alm, niter, x, nvars, lambda, nconstraints, log,
+                            OUT_diagnosis, NULL);
+}
+
+/* alm.c:324-328 (reads as many values as are present, zero-filling the
+ * rest; the original asserts on short input) */
+void read_doubles(FILE * stream, double * out, size_t n)
+{
+        for (size_t i = 0; i < n; i++)
+                if (fscanf(stream, " %lf", out + i) != 1) out[i] = 0;
+}
+
+static int read_exact(FILE * stream, double * out, size_t n)
+{
+        for (size_t i = 0; i < n; i++)
+                if (fscanf(stream, " %lf", out + i) != 1) return -1;
+        return 0;
+}
+
+/* alm.c:330-364 */
+alm_t * alm_read(FILE * stream)
+{
+        sparse_matrix_t * m = sparse_matrix_read(stream, 0);
+        if (m == NULL) return NULL;
+        size_t nvars = sparse_matrix_ncolumns(m), nrhs = sparse_matrix_nrows(m);
+        double * rhs = calloc(nrhs + 1, sizeof(double));
+        double * linear = calloc(nvars + 1, sizeof(double));
+        double * lower = calloc(nvars + 1, sizeof(double));
+        double * upper = calloc(nvars + 1, sizeof(double));
+        double * llo = calloc(nrhs + 1, sizeof(double));
+        double * lhi = calloc(nrhs + 1, sizeof(double));
+        alm_t * alm = NULL;
+        if (rhs && linear && lower && upper && llo && lhi
+            && !read_exact(stream, rhs, nrhs)
+            && !read_exact(stream, linear, nvars)
+            && !read_exact(stream, lower, nvars)
+            && !read_exact(stream, upper, nvars)
+            && !read_exact(stream, llo, nrhs)
+            && !read_exact(stream, lhi, nrhs))
+                alm = alm_make(m, nrhs, rhs, nvars, linear, lower, upper,
+                               llo, lhi);
+        if (alm == NULL) sparse_matrix_free(m);
+        free(rhs);
+        free(linear);
+        free(lower);
+        free(upper);
+        free(llo);
+        free(lhi);
+        return alm;
+}

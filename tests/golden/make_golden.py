@@ -1,0 +1,151 @@
+"""Write the golden fixtures under tests/golden/ from the reference.
+
+Runs ONLY where the reference sources are present: it builds the
+reference with oracle/Makefile (outputs in oracle/_ref/, never committed)
+and calls its public C interface through ctypes, on seeded instances from
+acdalm.instances.  Each fixture stores the instance digest, the call's
+arguments and its outputs; nothing of the reference's code is stored.
+
+    python tests/golden/make_golden.py
+
+Fixtures (numpy .npz, float64 unless noted):
+  spmv_<case>.npz   Z-ordered triplets after sparse_matrix_make (permute 0
+                    and 1), permutation images of arange, A x, A' y, and
+                    the two-RHS products for fixed x/y.
+  approx_<case>.npz approx_solve / mpg_solve from x = 0 for several
+                    iteration caps: return code, x, diagnosis, log text.
+  alm_<case>.npz    alm_solve (APPROX inner solver) from x = 0, lambda = 0:
+                    return code, x, lambda, diagnosis, log text.
+  cd_<case>.npz     cd_solve after srandom(1) (the as-shipped inner solver).
+  lp_optimum.npz    optimal objectives from scipy.optimize.linprog (HiGHS)
+                    for the small LPs, an independent pin of the optimum.
+"""
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "acd-alm_amd"), os.path.join(ROOT, "oracle")]
+
+from acdalm import instances as I  # noqa: E402
+from acdalm.capi import CLib  # noqa: E402
+import pyoracle  # noqa: E402
+
+LIBC = ctypes.CDLL(None)
+
+
+def spmv_cases():
+    return {
+        "small": I.random_lp(301, 555, 7, 12),
+        "dense": I.random_lp(19, 13, 9, 5),
+        "setcover": I.set_cover(300, 1200, 5),
+        "mcf": I.multicommodity(6, 3, 3),
+        "c0": I.config0(1),
+    }
+
+
+def solve_cases():
+    return {
+        "small": I.random_lp(60, 150, 4, 21),
+        "setcover": I.set_cover(80, 300, 6),
+        "mcf": I.multicommodity(4, 2, 7),
+        "c0": I.config0(1),
+    }
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, **arrays)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def main():
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"],
+                   check=True, stdout=subprocess.DEVNULL)
+    ref = CLib(pyoracle.REF_APPROX_SO, gpu_extras=False)
+    rng = np.random.default_rng(2024)
+    for name, lp in spmv_cases().items():
+        out = {"digest": np.array(lp.digest())}
+        x = rng.standard_normal(lp.ncols)
+        y = rng.standard_normal(lp.nrows)
+        x2 = rng.standard_normal(lp.ncols)
+        y2 = rng.standard_normal(lp.nrows)
+        out.update(x=x, y=y, x2=x2, y2=y2)
+        for perm in (0, 1):
+            m = ref.make_matrix(lp, perm)
+            r, c, v = ref.coo(m)
+            out[f"rows{perm}"], out[f"cols{perm}"], out[f"vals{perm}"] = r, c, v
+            out[f"rowperm{perm}"] = ref.permute(m, np.arange(lp.nrows, dtype=float), 1, 1)[1]
+            out[f"colperm{perm}"] = ref.permute(m, np.arange(lp.ncols, dtype=float), 0, 1)[1]
+            out[f"ax{perm}"] = ref.multiply(m, x, 0)
+            out[f"aty{perm}"] = ref.multiply(m, y, 1)
+            out[f"ax2_{perm}"] = np.stack(ref.multiply_2(m, x, x2, 0))
+            out[f"aty2_{perm}"] = np.stack(ref.multiply_2(m, y, y2, 1))
+            ref.free_matrix(m)
+        save(f"spmv_{name}.npz", **out)
+
+    for name, lp in solve_cases().items():
+        m = ref.make_matrix(lp, 0)
+        a = ref.approx_make(m, lp)
+        out = {"digest": np.array(lp.digest())}
+        caps = [1, 2, 9, 10, 99, 100, 101, 250] if name != "c0" else [1, 10, 100, 250]
+        out["caps"] = np.array(caps)
+        for solver in ("approx", "mpg"):
+            for cap in caps:
+                fn = ref.approx_solve if solver == "approx" else ref.mpg_solve
+                rc, x, diag, log = fn(a, np.zeros(lp.ncols), cap, log=True,
+                                      period=10, offset=0.5)
+                out[f"{solver}_rc_{cap}"] = np.array(rc)
+                out[f"{solver}_x_{cap}"] = x
+                out[f"{solver}_diag_{cap}"] = diag
+                out[f"{solver}_log_{cap}"] = np.array(log)
+        # stopping on the projected gradient
+        rc, x, diag, log = ref.approx_solve(a, np.zeros(lp.ncols), 100000,
+                                            max_pg=1e-3, min_delta=1e-12,
+                                            log=True, period=1000)
+        out.update(approx_pg_rc=np.array(rc), approx_pg_x=x,
+                   approx_pg_diag=diag, approx_pg_log=np.array(log))
+        save(f"approx_{name}.npz", **out)
+        ref.approx_free(a)
+
+        alm = ref.alm_make(m, lp)
+        rc, x, lam, diag, log = ref.alm_solve(alm, 1000, np.zeros(lp.ncols),
+                                              np.zeros(lp.nrows), log=True)
+        save(f"alm_{name}.npz", digest=np.array(lp.digest()), rc=np.array(rc),
+             x=x, lam=lam, diag=diag, log=np.array(log))
+        ref.alm_free(alm)
+        ref.free_matrix(m)
+
+    # the as-shipped inner solver on a small problem
+    lp = solve_cases()["small"]
+    m = ref.make_matrix(lp, 0)
+    a = ref.approx_make(m, lp)
+    LIBC.srandom(1)
+    rc, x, diag, log = ref.cd_solve(a, np.zeros(lp.ncols), 5, log=True,
+                                    period=10)
+    save("cd_small.npz", digest=np.array(lp.digest()), rc=np.array(rc), x=x,
+         diag=diag, log=np.array(log))
+    ref.approx_free(a)
+    ref.free_matrix(m)
+
+    from scipy.optimize import linprog
+    opt = {}
+    for name, lp in solve_cases().items():
+        A = lp.dense() if lp.nrows * lp.ncols < 2e7 else None
+        if A is None:
+            continue
+        bounds = list(zip(lp.lower, [None if u == np.inf else u for u in lp.upper]))
+        res = linprog(lp.linear, A_eq=A, b_eq=lp.rhs, bounds=bounds,
+                      method="highs")
+        assert res.status == 0, (name, res.message)
+        opt[name] = np.array(res.fun)
+        opt[name + "_x"] = res.x
+    save("lp_optimum.npz", **opt)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+"""GPU parity of the solvers (approx_solve, mpg_solve, alm_solve) against
+the reference's golden trajectories and the oracle.
+
+The FP64 GPU path performs the reference's arithmetic element by element
+(bit-identical products, step sizes, projections and momentum updates);
+only the order of the scalar reductions differs (fixed-order tree instead
+of two sequential lanes), so short trajectories agree to ~1e-12 and are
+checked at 1e-9.  Whole ALM solves stop on absolute thresholds whose
+crossing can move by one iteration under a 1-ulp change, so they are
+checked against the reference's final objective at 1e-6, and, with
+tightened stopping constants, against the LP optimum (HiGHS) and the
+oracle's tight objective at 1e-8 relative.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from acdalm import instances as I
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SOLVE = {"small": I.random_lp(60, 150, 4, 21), "setcover": I.set_cover(80, 300, 6),
+         "mcf": I.multicommodity(4, 2, 7), "c0": I.config0(1)}
+NUM = re.compile(r"[-+]?(?:\d+\.?\d*(?:e[-+]?\d+)?|inf|nan)")
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+def close(a, b, rtol):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    scale = max(1.0, float(np.max(np.abs(b)))) if b.size else 1.0
+    return a.shape == b.shape and bool(np.all(np.abs(a - b) <= rtol * scale))
+
+
+def logs_match(a, b, rtol=1e-5):
+    """Same lines and markers; numbers equal to the printed precision."""
+    la, lb = a.splitlines(), b.splitlines()
+    if len(la) != len(lb):
+        return False
+    for x, y in zip(la, lb):
+        if NUM.sub("#", x) != NUM.sub("#", y):
+            return False
+        for p, q in zip(NUM.findall(x), NUM.findall(y)):
+            fp, fq = float(p), float(q)
+            if not (fp == fq or abs(fp - fq) <= rtol * max(1.0, abs(fq))):
+                return False
+    return True
+
+
+@pytest.fixture(scope="module")
+def problems(gpu):
+    out = {}
+    for name, lp in SOLVE.items():
+        m = gpu.make_matrix(lp, 0)
+        out[name] = (lp, m, gpu.approx_make(m, lp))
+    yield out
+    for lp, m, a in out.values():
+        gpu.approx_free(a)
+        gpu.free_matrix(m)
+
+
+@pytest.mark.parametrize("name", sorted(SOLVE))
+@pytest.mark.parametrize("solver", ["approx", "mpg"])
+def test_inner_solver_matches_reference(gpu, problems, name, solver):
+    lp, m, a = problems[name]
+    f = load(f"approx_{name}.npz")
+    fn = gpu.approx_solve if solver == "approx" else gpu.mpg_solve
+    for cap in f["caps"]:
+        print(f"[solver] {solver} {name} niter={cap}", flush=True)
+        rc, x, diag, log = fn(a, np.zeros(lp.ncols), int(cap), log=True,
+                              period=10, offset=0.5)
+        assert rc == int(f[f"{solver}_rc_{cap}"])
+        assert close(x, f[f"{solver}_x_{cap}"], 1e-9), cap
+        assert close(diag[:4], f[f"{solver}_diag_{cap}"][:4], 1e-9), (diag, f[f"{solver}_diag_{cap}"])
+        assert diag[4] == f[f"{solver}_diag_{cap}"][4]
+        assert logs_match(log, str(f[f"{solver}_log_{cap}"])), (log, str(f[f"{solver}_log_{cap}"]))
+
+
+@pytest.mark.parametrize("name", ["small", "setcover", "mcf"])
+def test_approx_stops_on_projected_gradient(gpu, problems, name):
+    lp, m, a = problems[name]
+    f = load(f"approx_{name}.npz")
+    print(f"[solver] approx pg-stop {name}", flush=True)
+    rc, x, diag, log = gpu.approx_solve(a, np.zeros(lp.ncols), 100000,
+                                        max_pg=1e-3, min_delta=1e-12,
+                                        log=True, period=1000)
+    assert rc == int(f["approx_pg_rc"]) == 2
+    assert diag[2] < 1e-3
+    assert close(diag[0], f["approx_pg_diag"][0], 1e-6)
+
+
+@pytest.mark.parametrize("name", sorted(SOLVE))
+def test_alm_matches_reference(gpu, name):
+    lp = SOLVE[name]
+    f = load(f"alm_{name}.npz")
+    print(f"[solver] alm {name}", flush=True)
+    m = gpu.make_matrix(lp, 0)
+    alm = gpu.alm_make(m, lp)
+    rc, x, lam, diag, log = gpu.alm_solve(alm, 1000, np.zeros(lp.ncols),
+                                          np.zeros(lp.nrows), log=True)
+    gpu.alm_free(alm)
+    gpu.free_matrix(m)
+    assert rc == int(f["rc"]) == 0
+    ref = f["diag"]
+    assert abs(diag[0] - ref[0]) <= 1e-6 * max(1.0, abs(ref[0])), (diag, ref)
+    assert diag[1] < 1e-5 and ref[1] < 1e-5
+    # the first outer iteration is long enough to be identical in print
+    assert logs_match(log.splitlines()[0], str(f["log"]).splitlines()[0])
+
+
+@pytest.mark.parametrize("name", ["small", "setcover", "mcf"])
+def test_alm_tight_matches_optimum_and_oracle(gpu, oracle, name):
+    lp = SOLVE[name]
+    opt = float(load("lp_optimum.npz")[name])
+    tight = dict(stop_violation=1e-9, stop_pg=1e-9, precision_floor=1e-12)
+    print(f"[solver] alm tight {name}", flush=True)
+    m = gpu.make_matrix(lp, 0)
+    alm = gpu.alm_make(m, lp)
+    rc, x, lam, diag, _ = gpu.alm_solve_ex(alm, 5000, np.zeros(lp.ncols),
+                                           np.zeros(lp.nrows), **tight)
+    gpu.alm_free(alm)
+    gpu.free_matrix(m)
+    mo = oracle.make_matrix(lp, 0)
+    ao = oracle.alm_make(mo, lp)
+    rco, xo, lamo, diago, _ = oracle.alm_solve(ao, 5000, np.zeros(lp.ncols),
+                                               np.zeros(lp.nrows), **tight)
+    oracle.alm_free(ao)
+    oracle.free_matrix(mo)
+    assert rc == rco == 0
+    assert abs(diag[0] - diago[0]) <= 1e-8 * max(1.0, abs(diago[0])), (diag, diago)
+    assert abs(diag[0] - opt) <= 1e-6 * max(1.0, abs(opt))
+    infeas = np.max(np.abs(lp.dense() @ x - lp.rhs))
+    assert infeas <= 1e-8 * max(1.0, np.max(np.abs(lp.rhs)))
+
+
+def test_solves_are_deterministic(gpu, problems):
+    lp, m, a = problems["c0"]
+    print("[solver] determinism c0", flush=True)
+    r1 = gpu.approx_solve(a, np.zeros(lp.ncols), 300)
+    r2 = gpu.approx_solve(a, np.zeros(lp.ncols), 300)
+    assert r1[0] == r2[0]
+    assert np.array_equal(r1[1], r2[1]) and np.array_equal(r1[2], r2[2])
+
+
+def test_zero_iterations_and_value_stop(gpu, oracle, problems):
+    lp, m, a = problems["small"]
+    print("[solver] niter=0 / max_value", flush=True)
+    x0 = np.linspace(0, 1, lp.ncols)
+    rc, x, diag, _ = gpu.approx_solve(a, x0, 0)
+    mo = oracle.make_matrix(lp, 0)
+    ao = oracle.approx_make(mo, lp)
+    rco, xo, diago, _ = oracle.approx_solve(ao, x0, 0)
+    assert rc == rco == 0 and np.array_equal(x, xo)
+    assert close(diag, diago, 1e-12)
+    # a value threshold above the start stops at once with reason 1
+    rc, x, diag, _ = gpu.approx_solve(a, x0, 100, max_value=1e30)
+    rco, xo, diago, _ = oracle.approx_solve(ao, x0, 100, max_value=1e30)
+    assert rc == rco == 1 and diag[4] == diago[4]
+    # a movement threshold that is always met stops at iteration 100
+    rc, x, diag, _ = gpu.approx_solve(a, x0, 1000, min_delta=1e30)
+    rco, xo, diago, _ = oracle.approx_solve(ao, x0, 1000, min_delta=1e30)
+    assert rc == rco == 3 and diag[4] == diago[4] == 100
+    assert close(x, xo, 1e-9)
+    oracle.approx_free(ao)
+    oracle.free_matrix(mo)

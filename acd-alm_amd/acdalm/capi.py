@@ -176,6 +176,13 @@ class CLib:
             ("alm_solve_ex", C.c_int, [vp, sz, dp, sz, dp, sz, vp, dp,
                                        C.POINTER(AlmOptions)]),
             ("acd_last_solve_stats", None, [C.POINTER(SolveStats)]),
+            ("acd_alm_reset", C.c_int, [vp]),
+            ("acd_spmv_variant_time", C.c_int, [vp, C.c_int, vp, vp, C.c_int,
+                                                C.c_int, C.POINTER(C.c_double)]),
+            ("acd_spmv_set_variant", C.c_int, [C.c_int]),
+            ("acd_last_solve_profile", None, [C.POINTER(C.c_uint64),
+                                              C.POINTER(C.c_double), C.c_int]),
+            ("acd_pass_name", C.c_char_p, [C.c_int]),
         ]:
             fn = getattr(L, name, None)
             if fn is None:
@@ -323,6 +330,17 @@ class CLib:
         rc = self.lib.alm_solve_ex(a, niter, _d(x), x.shape[0], _d(lam),
                                    lam.shape[0], cap.fp, _d(diag), C.byref(o))
         return rc, x, lam, diag, cap.close()
+
+    def profile(self):
+        """{pass name: (launches, microseconds)} of the last solve."""
+        cnt = (C.c_uint64 * 32)()
+        us = (C.c_double * 32)()
+        self.lib.acd_last_solve_profile(cnt, us, 32)
+        out = {}
+        for i in range(32):
+            if cnt[i]:
+                out[self.lib.acd_pass_name(i).decode()] = (int(cnt[i]), float(us[i]))
+        return out
 
     def stats(self):
         st = SolveStats()

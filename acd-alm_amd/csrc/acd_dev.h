@@ -92,6 +92,10 @@ int acd_dev_gather(double * dest, const double * src, const uint32_t * p,
 int acd_dev_spmv_time(const struct acd_dev_matrix *, int transpose,
                       double * y, const double * x, int reps,
                       double * OUT_ms);
+int acd_dev_spmv_time_variant(const struct acd_dev_matrix *, int transpose,
+                              double * y, const double * x, int reps,
+                              int variant, double * OUT_ms);
+int acd_dev_spmv_set_default_variant(int variant);
 
 /* ---- device-resident solver (dev_solver.hip) ---- */
 /* Original-order vectors of a device problem that the host uploads. */
@@ -167,6 +171,9 @@ double * acd_dev_lp_vec(struct acd_dev_lp *, int which);
 int acd_dev_approx_update(struct acd_dev_lp *);
 int acd_dev_solve(struct acd_dev_lp *, const struct acd_solve_request *,
                   struct acd_solve_result *, acd_event_sink, void * ctx);
+/* Per-op launch counts and microseconds (step + control) of the last
+ * solve, indexed by op code. */
+void acd_dev_last_profile(uint64_t * counts, double * us, int n);
 
 #ifdef __cplusplus
 }

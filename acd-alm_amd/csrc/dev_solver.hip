@@ -1,14 +1,19 @@
-// dev_solver.hip -- the device-resident LP solver behind approx_solve,
-// mpg_solve and alm_solve: one cooperative (co-resident) grid per LP
-// running the state machine of solver_core.hpp, relaunched in chunks of
-// iterations until the solve ends.  Between chunks the host only reads
-// the small state record and replays log events.
+// dev_solver.hip -- device-resident LP solver for one large LP: the ops of
+// solver_ops.hpp run as a wide "step" kernel (one op per launch, each
+// workgroup leaving its partial reductions in memory) followed by a
+// one-workgroup "control" kernel that combines the partials in block
+// order and advances the state machine.  The kernel boundary is the
+// global barrier.  Pairs of (step, control) launches are captured once per
+// problem in a hipGraph and replayed in chunks; between chunks the host
+// reads the small state record, replays log events and stops when the
+// solve is done.  Only the control kernel takes decisions, so the host
+// never participates in an iteration.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include "acd_dev.h"
-#include "solver_core.hpp"
+#include "solver_ops.hpp"
 
 using namespace acd;
 
@@ -16,13 +21,89 @@ namespace {
 
 inline hipStream_t S() { return (hipStream_t)acd_dev_stream(); }
 
-__global__ void __launch_bounds__(kBlock) k_solver(Args a) {
-  GridGroup g;
-  g.init(a.sa);
-  Solver<GridGroup> sv(g, a);
-  sv.s = *a.st;
-  sv.run();
-  if (blockIdx.x == 0 && threadIdx.x == 0) *a.st = sv.s;
+constexpr int kWPB = 4;  // wavefronts per workgroup of the step kernel
+constexpr uint32_t kEventCap = 4096;
+constexpr int kPairs = 32;  // (step, control) pairs per graph
+uint64_t g_prof_count[32];
+double g_prof_us[32];
+
+struct GridG {
+  uint32_t lane, gwave, nwaves;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; renumber them so
+// that each XCD (and its L2) walks a contiguous range of slices.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  if (nb % 8) return b;
+  return (b % 8) * (nb / 8) + b / 8;
+}
+
+__global__ void __launch_bounds__(64 * kWPB) k_step(Args a) {
+  const State& s = *a.st;
+  if (s.op == OP_NONE) return;
+  GridG g;
+  g.lane = threadIdx.x & 63;
+  g.gwave = xcd_block(blockIdx.x, gridDim.x) * kWPB + (threadIdx.x >> 6);
+  g.nwaves = gridDim.x * kWPB;
+  Ops<GridG> ops(g, a, s);
+  ops.run();
+  const uint32_t mm = s.op_maxmask;
+  __shared__ double lds[kWPB][kRes];
+#pragma unroll
+  for (int k = 0; k < kRes; k++) {
+    const double v = wave_reduce(ops.p[k], (mm >> k) & 1);
+    if (g.lane == 0) lds[threadIdx.x >> 6][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kRes) {
+    const int k = threadIdx.x;
+    const bool mx = (mm >> k) & 1;
+    double acc = lds[0][k];
+    for (int w = 1; w < kWPB; w++) acc = combine(acc, lds[w][k], mx);
+    a.partials[(size_t)blockIdx.x * kRes + k] = acc;
+  }
+}
+
+// Combines the step kernel's per-block partials (wave w owns slots 2w and
+// 2w+1; lane l sums blocks l, l+64, ... in order, then the lanes combine
+// by butterfly), then advances the state machine on one thread.
+__global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
+  __shared__ double res[kRes];
+  State* st = a.st;
+  const uint32_t mm = st->op_maxmask;
+  const bool ran = st->op != OP_NONE;
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++) {
+    const int k = 2 * w + kk;
+    const bool mx = (mm >> k) & 1;
+    double acc = mx ? -HUGE_VAL : 0.0;
+    if (ran) {
+      bool first = true;
+      for (uint32_t b = lane; b < nblocks; b += 64) {
+        const double v = a.partials[(size_t)b * kRes + k];
+        acc = first ? v : combine(acc, v, mx);
+        first = false;
+      }
+      if (first) acc = mx ? -HUGE_VAL : 0.0;
+    }
+    acc = wave_reduce(acc, mx);
+    if (lane == 0) res[k] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && st->pc != PC_DONE && !st->error) {
+    const uint64_t now = wall_clock64();
+    const int32_t op = st->op;
+    if (ran && op >= 0 && op < 32) {
+      st->op_count[op]++;
+      st->op_ticks[op] += now - st->t_last;
+    }
+    State s = *st;
+    Ctl c{s, a, res};
+    c.advance();
+    s.t_last = wall_clock64();
+    *st = s;
+  }
 }
 
 Sell view(const acd_sell& op) {
@@ -34,8 +115,6 @@ Sell view(const acd_sell& op) {
   v.nslices = op.nslices;
   return v;
 }
-
-constexpr uint32_t kEventCap = 4096;
 
 }  // namespace
 
@@ -49,86 +128,16 @@ struct acd_dev_lp {
   double* vn[kNumVN];
   double* vm[kNumVM];
   State* st;
-  SyncArea* sa;
+  double* partials;
   acd_event* ev;
-  int max_blocks;  // co-resident workgroups of k_solver
+  int blocks;
+  hipGraphExec_t graph;
+  int have_graph;
 };
-
-extern "C" {
-
-struct acd_dev_lp* acd_dev_lp_create(const struct acd_dev_matrix* orig,
-                                     const struct acd_dev_matrix* perm) {
-  if (acd_dev_init()) return nullptr;
-  acd_dev_lp* lp = (acd_dev_lp*)calloc(1, sizeof(acd_dev_lp));
-  if (!lp) return nullptr;
-  lp->orig = orig;
-  lp->perm = perm;
-  lp->m = orig->nrows;
-  lp->n = orig->ncols;
-  size_t m = lp->m + 1, n = lp->n + 1;
-  bool ok = true;
-  auto al = [&](size_t count) -> double* {
-    double* p = (double*)acd_dev_malloc(count * sizeof(double));
-    if (!p) ok = false;
-    return p;
-  };
-  const int is_m[ACD_V_COUNT] = {1, 1, 0, 0, 0, 0, 1, 1, 0, 1};
-  for (int v = 0; v < ACD_V_COUNT; v++) lp->vec[v] = al(is_m[v] ? m : n);
-  lp->r = al(m);
-  lp->p_rhs = al(m);
-  lp->p_w = al(m);
-  lp->p_c = al(n);
-  lp->p_l = al(n);
-  lp->p_u = al(n);
-  lp->p_v = al(n);
-  lp->p_iv = al(n);
-  for (int i = 0; i < kNumVN; i++) lp->vn[i] = al(n);
-  for (int i = 0; i < kNumVM; i++) lp->vm[i] = al(m);
-  lp->st = (State*)acd_dev_malloc(sizeof(State));
-  lp->sa = (SyncArea*)acd_dev_malloc(sizeof(SyncArea));
-  lp->ev = (acd_event*)acd_dev_malloc(sizeof(acd_event) * kEventCap);
-  if (!ok || !lp->st || !lp->sa || !lp->ev) {
-    acd_dev_lp_free(lp);
-    return nullptr;
-  }
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_solver, kBlock,
-                                                   0) != hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
-  lp->max_blocks = per_cu * acd_dev_cu_count();
-  if (lp->max_blocks > kMaxBlocks) lp->max_blocks = kMaxBlocks;
-  return lp;
-}
-
-void acd_dev_lp_free(struct acd_dev_lp* lp) {
-  if (!lp) return;
-  for (int v = 0; v < ACD_V_COUNT; v++) acd_dev_free(lp->vec[v]);
-  acd_dev_free(lp->r);
-  acd_dev_free(lp->p_rhs);
-  acd_dev_free(lp->p_w);
-  acd_dev_free(lp->p_c);
-  acd_dev_free(lp->p_l);
-  acd_dev_free(lp->p_u);
-  acd_dev_free(lp->p_v);
-  acd_dev_free(lp->p_iv);
-  for (int i = 0; i < kNumVN; i++) acd_dev_free(lp->vn[i]);
-  for (int i = 0; i < kNumVM; i++) acd_dev_free(lp->vm[i]);
-  acd_dev_free(lp->st);
-  acd_dev_free(lp->sa);
-  acd_dev_free(lp->ev);
-  free(lp);
-}
-
-double* acd_dev_lp_vec(struct acd_dev_lp* lp, int which) {
-  return (which >= 0 && which < ACD_V_COUNT) ? lp->vec[which] : nullptr;
-}
-
-}  // extern "C"
 
 namespace {
 
-Args make_args(acd_dev_lp* lp, uint64_t budget) {
+Args make_args(acd_dev_lp* lp) {
   Args a;
   memset(&a, 0, sizeof a);
   a.m = lp->m;
@@ -160,35 +169,56 @@ Args make_args(acd_dev_lp* lp, uint64_t budget) {
   a.p_v = lp->p_v;
   a.p_iv = lp->p_iv;
   a.st = lp->st;
-  a.sa = lp->sa;
+  a.partials = lp->partials;
   a.ev = lp->ev;
   a.ev_cap = kEventCap;
-  a.budget = budget;
   return a;
 }
 
-// Workgroups for this LP: enough waves for the largest operator, within
-// what can be co-resident.
-int pick_blocks(const acd_dev_lp* lp) {
-  uint32_t slices = lp->perm->at.nslices;
-  if (lp->perm->a.nslices > slices) slices = lp->perm->a.nslices;
-  if (lp->orig->at.nslices > slices) slices = lp->orig->at.nslices;
-  if (lp->orig->a.nslices > slices) slices = lp->orig->a.nslices;
+// Enough wavefronts for every slice of the largest operator, capped at
+// what the chip keeps resident (grid-stride beyond that).
+int pick_blocks(const acd_dev_matrix* orig, const acd_dev_matrix* perm) {
+  uint32_t slices = perm->at.nslices;
+  if (perm->a.nslices > slices) slices = perm->a.nslices;
+  if (orig->at.nslices > slices) slices = orig->at.nslices;
+  if (orig->a.nslices > slices) slices = orig->a.nslices;
   int want = (int)((slices + kWPB - 1) / kWPB);
+  int cap = 8 * acd_dev_cu_count();
+  if (want > 64) want = (want + 7) / 8 * 8;  // whole XCD rounds
+  if (cap < 64) cap = 64;
+  if (want > cap) want = cap;
   if (want < 1) want = 1;
-  if (want > lp->max_blocks) want = lp->max_blocks;
   return want;
 }
 
-int launch(acd_dev_lp* lp, uint64_t budget, int blocks) {
-  Args a = make_args(lp, budget);
-  void* params[] = {&a};
-  hipError_t e = hipLaunchCooperativeKernel((const void*)k_solver, dim3(blocks),
-                                            dim3(kBlock), params, 0, S());
-  if (e != hipSuccess) {
-    acd_dev_record_error((int)e, "cooperative launch of the solver");
-    return (int)e;
+int build_graph(acd_dev_lp* lp) {
+  if (lp->have_graph) return 0;
+  hipStream_t cs;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+    return -1;
+  Args a = make_args(lp);
+  hipGraph_t g;
+  int rc = 0;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) !=
+      hipSuccess) {
+    rc = -1;
+  } else {
+    for (int i = 0; i < kPairs; i++) {
+      k_step<<<lp->blocks, 64 * kWPB, 0, cs>>>(a);
+      k_ctl<<<1, 256, 0, cs>>>(a, (uint32_t)lp->blocks);
+    }
+    if (hipStreamEndCapture(cs, &g) != hipSuccess) rc = -1;
+    if (!rc && hipGraphInstantiate(&lp->graph, g, nullptr, nullptr, 0) !=
+                   hipSuccess)
+      rc = -1;
+    if (!rc) hipGraphDestroy(g);
   }
+  hipStreamDestroy(cs);
+  if (rc) {
+    acd_dev_record_error(rc, "solver graph capture");
+    return rc;
+  }
+  lp->have_graph = 1;
   return 0;
 }
 
@@ -210,36 +240,43 @@ void init_state(State* s, const acd_dev_lp* lp, const acd_solve_request* q) {
   s->stop_pg = q->stop_pg;
   s->precision_floor = q->precision_floor;
   s->inner_min_delta = q->inner_min_delta;
+  s->op = OP_NONE;
   switch (q->mode) {
-    case ACD_MODE_APPROX: s->phase = PH_APPROX_INIT; break;
-    case ACD_MODE_MPG: s->phase = PH_MPG_INIT; break;
+    case ACD_MODE_APPROX: s->pc = PC_A_INIT; break;
+    case ACD_MODE_MPG: s->pc = PC_M0; break;
     case ACD_MODE_ALM:
-      s->phase = PH_ALM_START;
+      s->pc = PC_L0;
       s->outer_niter = q->niter;
       break;
-    default: s->phase = PH_UPDATE_ONLY; break;
+    default: s->pc = PC_U0; break;
   }
 }
 
 int run_to_completion(acd_dev_lp* lp, State* hs, acd_solve_result* res,
-                      acd_event_sink sink, void* ctx, uint64_t budget) {
-  const int blocks = pick_blocks(lp);
+                      acd_event_sink sink, void* ctx) {
+  if (build_graph(lp)) return -1;
+  Args a = make_args(lp);
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   acd_event* evbuf = (acd_event*)malloc(sizeof(acd_event) * kEventCap);
   int rc = 0;
-  double ms_total = 0;
   uint64_t launches = 0;
   if (hipMemcpyAsync(lp->st, hs, sizeof(State), hipMemcpyHostToDevice, S()) !=
-          hipSuccess ||
-      hipMemsetAsync(lp->sa, 0, sizeof(uint32_t) * 4, S()) != hipSuccess) {
+      hipSuccess)
     rc = -1;
-  }
+  hipEventRecord(e0, S());
+  if (!rc) k_ctl<<<1, 256, 0, S()>>>(a, (uint32_t)lp->blocks);
+  int chunk = 1;
   while (!rc) {
-    hipEventRecord(e0, S());
-    if ((rc = launch(lp, budget, blocks))) break;
-    hipEventRecord(e1, S());
+    for (int c = 0; c < chunk && !rc; c++) {
+      if (hipGraphLaunch(lp->graph, S()) != hipSuccess) rc = -1;
+      launches += 2 * kPairs;
+    }
+    if (rc) {
+      acd_dev_record_error(rc, "solver graph launch");
+      break;
+    }
     if (hipMemcpyAsync(hs, lp->st, sizeof(State), hipMemcpyDeviceToHost,
                        S()) != hipSuccess ||
         hipStreamSynchronize(S()) != hipSuccess) {
@@ -247,29 +284,41 @@ int run_to_completion(acd_dev_lp* lp, State* hs, acd_solve_result* res,
       rc = -1;
       break;
     }
-    float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
-    ms_total += ms;
-    launches++;
     if (hs->events) {
-      uint64_t ne = hs->events < kEventCap ? hs->events : kEventCap;
+      const uint64_t ne = hs->events < kEventCap ? hs->events : kEventCap;
       hipMemcpy(evbuf, lp->ev, sizeof(acd_event) * ne, hipMemcpyDeviceToHost);
       if (sink)
         for (uint64_t i = 0; i < ne; i++) sink(ctx, &evbuf[i]);
       hs->events = 0;
+      hs->events_full = 0;
       hipMemcpyAsync(&lp->st->events, &hs->events, sizeof(uint64_t),
                      hipMemcpyHostToDevice, S());
+      hipMemcpyAsync(&lp->st->events_full, &hs->events_full, sizeof(int32_t),
+                     hipMemcpyHostToDevice, S());
     }
-    if (hs->error || hs->phase == PH_DONE) break;
+    if (hs->error || hs->pc == PC_DONE) break;
+    if (chunk < 64) chunk *= 2;
   }
-  hipStreamSynchronize(S());
+  hipEventRecord(e1, S());
+  hipEventSynchronize(e1);
+  float ms = 0;
+  hipEventElapsedTime(&ms, e0, e1);
   free(evbuf);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  {
+    int khz = 100000;
+    hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate,
+                          acd_dev_device());
+    for (int i = 0; i < 32; i++) {
+      g_prof_count[i] = hs->op_count[i];
+      g_prof_us[i] = khz > 0 ? 1e3 * (double)hs->op_ticks[i] / khz : 0.0;
+    }
+  }
   if (res) {
     res->error = hs->error ? hs->error : (rc ? ACD_ERR_LAUNCH : 0);
     res->launches = launches;
-    res->device_ms = ms_total;
+    res->device_ms = ms;
     res->inner_iterations = hs->inner_total;
     res->outer_iterations = hs->outer;
     if (hs->mode == ACD_MODE_ALM) {
@@ -288,14 +337,83 @@ int run_to_completion(acd_dev_lp* lp, State* hs, acd_solve_result* res,
 
 extern "C" {
 
+void acd_dev_last_profile(uint64_t* counts, double* us, int n) {
+  for (int i = 0; i < n && i < 32; i++) {
+    counts[i] = g_prof_count[i];
+    us[i] = g_prof_us[i];
+  }
+}
+
+struct acd_dev_lp* acd_dev_lp_create(const struct acd_dev_matrix* orig,
+                                     const struct acd_dev_matrix* perm) {
+  if (acd_dev_init()) return nullptr;
+  acd_dev_lp* lp = (acd_dev_lp*)calloc(1, sizeof(acd_dev_lp));
+  if (!lp) return nullptr;
+  lp->orig = orig;
+  lp->perm = perm;
+  lp->m = orig->nrows;
+  lp->n = orig->ncols;
+  size_t m = lp->m + 1, n = lp->n + 1;
+  bool ok = true;
+  auto al = [&](size_t count) -> double* {
+    double* p = (double*)acd_dev_malloc(count * sizeof(double));
+    if (!p) ok = false;
+    return p;
+  };
+  const int is_m[ACD_V_COUNT] = {1, 1, 0, 0, 0, 0, 1, 1, 0, 1};
+  for (int v = 0; v < ACD_V_COUNT; v++) lp->vec[v] = al(is_m[v] ? m : n);
+  lp->r = al(m);
+  lp->p_rhs = al(m);
+  lp->p_w = al(m);
+  lp->p_c = al(n);
+  lp->p_l = al(n);
+  lp->p_u = al(n);
+  lp->p_v = al(n);
+  lp->p_iv = al(n);
+  for (int i = 0; i < kNumVN; i++) lp->vn[i] = al(n);
+  for (int i = 0; i < kNumVM; i++) lp->vm[i] = al(m);
+  lp->blocks = pick_blocks(orig, perm);
+  lp->partials = al((size_t)lp->blocks * kRes);
+  lp->st = (State*)acd_dev_malloc(sizeof(State));
+  lp->ev = (acd_event*)acd_dev_malloc(sizeof(acd_event) * kEventCap);
+  if (!ok || !lp->st || !lp->ev) {
+    acd_dev_lp_free(lp);
+    return nullptr;
+  }
+  return lp;
+}
+
+void acd_dev_lp_free(struct acd_dev_lp* lp) {
+  if (!lp) return;
+  if (lp->have_graph) hipGraphExecDestroy(lp->graph);
+  for (int v = 0; v < ACD_V_COUNT; v++) acd_dev_free(lp->vec[v]);
+  acd_dev_free(lp->r);
+  acd_dev_free(lp->p_rhs);
+  acd_dev_free(lp->p_w);
+  acd_dev_free(lp->p_c);
+  acd_dev_free(lp->p_l);
+  acd_dev_free(lp->p_u);
+  acd_dev_free(lp->p_v);
+  acd_dev_free(lp->p_iv);
+  for (int i = 0; i < kNumVN; i++) acd_dev_free(lp->vn[i]);
+  for (int i = 0; i < kNumVM; i++) acd_dev_free(lp->vm[i]);
+  acd_dev_free(lp->partials);
+  acd_dev_free(lp->st);
+  acd_dev_free(lp->ev);
+  free(lp);
+}
+
+double* acd_dev_lp_vec(struct acd_dev_lp* lp, int which) {
+  return (which >= 0 && which < ACD_V_COUNT) ? lp->vec[which] : nullptr;
+}
+
 int acd_dev_approx_update(struct acd_dev_lp* lp) {
   acd_solve_request q;
   memset(&q, 0, sizeof q);
-  q.mode = 0;
   State hs;
   init_state(&hs, lp, &q);
   acd_solve_result res;
-  return run_to_completion(lp, &hs, &res, nullptr, nullptr, 1);
+  return run_to_completion(lp, &hs, &res, nullptr, nullptr);
 }
 
 int acd_dev_solve(struct acd_dev_lp* lp, const struct acd_solve_request* q,
@@ -303,8 +421,7 @@ int acd_dev_solve(struct acd_dev_lp* lp, const struct acd_solve_request* q,
                   void* ctx) {
   State hs;
   init_state(&hs, lp, q);
-  const uint64_t budget = q->log ? 2000 : 20000;
-  return run_to_completion(lp, &hs, res, sink, ctx, budget);
+  return run_to_completion(lp, &hs, res, sink, ctx);
 }
 
 }  // extern "C"

@@ -3,13 +3,18 @@
 //
 // Replaces mult_subblock / mult2_subblock (spmv/spmv_block.c:160-254) and
 // the thread_pool_for dispatch of sparse_matrix_multiply(_2)
-// (spmv/spmv.c:159-233) of pkhuong/acd-alm.  One wavefront owns one
-// 64-row slice and one lane owns one row: the lane adds value*x[column]
-// over its row in ascending column order, starting from +0.0, with a
-// separate multiply and add (the build disables FP contraction), which is
-// the reference's per-row sum bit for bit.  Value and index streams are
-// read as coalesced 64-lane columns; four columns are loaded ahead so each
-// lane keeps four gathers in flight.
+// (spmv/spmv.c:159-233) of pkhuong/acd-alm.  One lane owns one row and
+// adds value*x[column] over its row in ascending column order, starting
+// from +0.0, with a separate multiply and add (the build disables FP
+// contraction): the reference's per-row sum bit for bit.  Value and index
+// streams are read as coalesced 64-lane columns of a 64-row slice.
+//
+// Because the per-row sum is sequential, throughput comes from memory-
+// level parallelism: a lane loads U columns of its slice (values and
+// indices) at once, then issues the U gathers, then adds in order; and a
+// wavefront can interleave R slices so R independent rows are in flight
+// per lane.  acd_dev_spmv_variant() exposes the (U, R) choices for tuning;
+// the default is the measured best on MI355X.
 #include <hip/hip_runtime.h>
 #include "acd_dev.h"
 
@@ -19,12 +24,105 @@ inline hipStream_t S() { return (hipStream_t)acd_dev_stream(); }
 
 constexpr int kWaves = 4;  // wavefronts per workgroup
 
-template <int NRHS>
+template <int U, bool NT>
+__device__ __forceinline__ double row_dot(const double* __restrict__ v,
+                                          const uint32_t* __restrict__ c,
+                                          uint32_t len,
+                                          const double* __restrict__ x) {
+  double acc = 0.0;
+  uint32_t k = 0;
+  for (; k + U <= len; k += U, v += 64 * U, c += 64 * U) {
+    double vv[U];
+    uint32_t cc[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      vv[u] = NT ? __builtin_nontemporal_load(v + 64 * u) : v[64 * u];
+      cc[u] = NT ? __builtin_nontemporal_load(c + 64 * u) : c[64 * u];
+    }
+    double xx[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) xx[u] = x[cc[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++) acc = acc + vv[u] * xx[u];
+  }
+  for (; k < len; k++, v += 64, c += 64) acc = acc + (*v) * x[*c];
+  return acc;
+}
+
+// R slices per wavefront, interleaved chunk by chunk so R independent
+// row sums are in flight per lane.
+template <int U, int R, bool NT>
 __global__ void __launch_bounds__(64 * kWaves)
-k_spmv(const uint64_t* __restrict__ off, const uint32_t* __restrict__ col,
-       const double* __restrict__ val, uint32_t nrows, uint32_t nslices,
-       const double* __restrict__ x0, const double* __restrict__ x1,
-       double* __restrict__ y0, double* __restrict__ y1) {
+k_spmv1(const uint64_t* __restrict__ off, const uint32_t* __restrict__ col,
+        const double* __restrict__ val, uint32_t nrows, uint32_t nslices,
+        const double* __restrict__ x, double* __restrict__ y) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t s0 = (blockIdx.x * kWaves + threadIdx.x / 64) * R;
+  if (R == 1) {
+    if (s0 >= nslices) return;
+    const uint64_t base = off[s0];
+    const uint32_t len = (uint32_t)((off[s0 + 1] - base) >> 6);
+    const double a = row_dot<U, NT>(val + base + lane, col + base + lane, len, x);
+    const uint32_t r = s0 * 64 + lane;
+    if (r < nrows) y[r] = a;
+    return;
+  }
+  const double* v[R];
+  const uint32_t* c[R];
+  uint32_t len[R];
+  double acc[R];
+  uint32_t maxlen = 0;
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    const uint32_t s = s0 + q;
+    uint64_t base = 0;
+    len[q] = 0;
+    if (s < nslices) {
+      base = off[s];
+      len[q] = (uint32_t)((off[s + 1] - base) >> 6);
+    }
+    v[q] = val + base + lane;
+    c[q] = col + base + lane;
+    acc[q] = 0.0;
+    maxlen = len[q] > maxlen ? len[q] : maxlen;
+  }
+  for (uint32_t k = 0; k < maxlen; k += U) {
+    double vv[R][U], xx[R][U];
+    uint32_t cc[R][U];
+#pragma unroll
+    for (int q = 0; q < R; q++)
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const bool in = k + u < len[q];
+        vv[q][u] = in ? (NT ? __builtin_nontemporal_load(v[q] + 64 * u) : v[q][64 * u]) : 0.0;
+        cc[q][u] = in ? (NT ? __builtin_nontemporal_load(c[q] + 64 * u) : c[q][64 * u]) : 0u;
+      }
+#pragma unroll
+    for (int q = 0; q < R; q++)
+#pragma unroll
+      for (int u = 0; u < U; u++) xx[q][u] = k + u < len[q] ? x[cc[q][u]] : 0.0;
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (k + u < len[q]) acc[q] = acc[q] + vv[q][u] * xx[q][u];
+      v[q] += 64 * U;
+      c[q] += 64 * U;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    const uint32_t r = (s0 + q) * 64 + lane;
+    if (s0 + q < nslices && r < nrows) y[r] = acc[q];
+  }
+}
+
+template <int U>
+__global__ void __launch_bounds__(64 * kWaves)
+k_spmv2(const uint64_t* __restrict__ off, const uint32_t* __restrict__ col,
+        const double* __restrict__ val, uint32_t nrows, uint32_t nslices,
+        const double* __restrict__ x0, const double* __restrict__ x1,
+        double* __restrict__ y0, double* __restrict__ y1) {
   const uint32_t s = blockIdx.x * kWaves + threadIdx.x / 64;
   const uint32_t lane = threadIdx.x & 63;
   if (s >= nslices) return;
@@ -34,38 +132,35 @@ k_spmv(const uint64_t* __restrict__ off, const uint32_t* __restrict__ col,
   const uint32_t* c = col + base + lane;
   double a0 = 0.0, a1 = 0.0;
   uint32_t k = 0;
-  for (; k + 4 <= len; k += 4, v += 256, c += 256) {
-    const double v0 = __builtin_nontemporal_load(v);
-    const double v1 = __builtin_nontemporal_load(v + 64);
-    const double v2 = __builtin_nontemporal_load(v + 128);
-    const double v3 = __builtin_nontemporal_load(v + 192);
-    const uint32_t c0 = __builtin_nontemporal_load(c);
-    const uint32_t c1 = __builtin_nontemporal_load(c + 64);
-    const uint32_t c2 = __builtin_nontemporal_load(c + 128);
-    const uint32_t c3 = __builtin_nontemporal_load(c + 192);
-    const double p0 = x0[c0], p1 = x0[c1], p2 = x0[c2], p3 = x0[c3];
-    a0 = a0 + v0 * p0;
-    a0 = a0 + v1 * p1;
-    a0 = a0 + v2 * p2;
-    a0 = a0 + v3 * p3;
-    if (NRHS == 2) {
-      const double q0 = x1[c0], q1 = x1[c1], q2 = x1[c2], q3 = x1[c3];
-      a1 = a1 + v0 * q0;
-      a1 = a1 + v1 * q1;
-      a1 = a1 + v2 * q2;
-      a1 = a1 + v3 * q3;
+  for (; k + U <= len; k += U, v += 64 * U, c += 64 * U) {
+    double vv[U], p[U], q[U];
+    uint32_t cc[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      vv[u] = __builtin_nontemporal_load(v + 64 * u);
+      cc[u] = __builtin_nontemporal_load(c + 64 * u);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      p[u] = x0[cc[u]];
+      q[u] = x1[cc[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      a0 = a0 + vv[u] * p[u];
+      a1 = a1 + vv[u] * q[u];
     }
   }
   for (; k < len; k++, v += 64, c += 64) {
     const double vv = *v;
     const uint32_t cc = *c;
     a0 = a0 + vv * x0[cc];
-    if (NRHS == 2) a1 = a1 + vv * x1[cc];
+    a1 = a1 + vv * x1[cc];
   }
   const uint32_t r = s * 64 + lane;
   if (r < nrows) {
     y0[r] = a0;
-    if (NRHS == 2) y1[r] = a1;
+    y1[r] = a1;
   }
 }
 
@@ -79,21 +174,42 @@ const struct acd_sell* pick(const struct acd_dev_matrix* m, int transpose) {
   return transpose ? &m->at : &m->a;
 }
 
-template <int NRHS>
-int launch(const struct acd_sell* op, double* y0, double* y1, const double* x0,
-           const double* x1) {
-  if (op->nslices == 0) return 0;
-  unsigned grid = (op->nslices + kWaves - 1) / kWaves;
-  k_spmv<NRHS><<<grid, 64 * kWaves, 0, S()>>>(op->off, op->col, op->val,
-                                              op->nrows, op->nslices, x0, x1,
-                                              y0, y1);
+int check(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
-    acd_dev_record_error((int)e, "spmv launch");
+    acd_dev_record_error((int)e, what);
     return (int)e;
   }
   return 0;
 }
+
+template <int U, int R, bool NT>
+int launch1_t(const struct acd_sell* op, double* y, const double* x) {
+  const uint32_t waves = (op->nslices + R - 1) / R;
+  const unsigned grid = (waves + kWaves - 1) / kWaves;
+  k_spmv1<U, R, NT><<<grid, 64 * kWaves, 0, S()>>>(op->off, op->col, op->val,
+                                                   op->nrows, op->nslices, x, y);
+  return check("spmv launch");
+}
+
+// Variant table: (unroll, slices per wave, nontemporal loads).
+int launch1(const struct acd_sell* op, double* y, const double* x, int variant) {
+  if (op->nslices == 0) return 0;
+  switch (variant) {
+    case 0: return launch1_t<4, 1, true>(op, y, x);
+    case 1: return launch1_t<4, 1, false>(op, y, x);
+    case 2: return launch1_t<8, 1, true>(op, y, x);
+    case 3: return launch1_t<8, 1, false>(op, y, x);
+    case 4: return launch1_t<4, 2, true>(op, y, x);
+    case 5: return launch1_t<8, 2, true>(op, y, x);
+    case 6: return launch1_t<4, 4, true>(op, y, x);
+    case 7: return launch1_t<16, 1, true>(op, y, x);
+    case 8: return launch1_t<2, 4, true>(op, y, x);
+    default: return launch1_t<4, 1, true>(op, y, x);
+  }
+}
+
+int g_default_variant = 0;
 
 }  // namespace
 
@@ -101,35 +217,40 @@ extern "C" {
 
 int acd_dev_spmv(const struct acd_dev_matrix* m, int transpose, double* y,
                  const double* x) {
-  return launch<1>(pick(m, transpose), y, nullptr, x, nullptr);
+  return launch1(pick(m, transpose), y, x, g_default_variant);
 }
 
 int acd_dev_spmv2(const struct acd_dev_matrix* m, int transpose, double* y0,
                   double* y1, const double* x0, const double* x1) {
-  return launch<2>(pick(m, transpose), y0, y1, x0, x1);
+  const struct acd_sell* op = pick(m, transpose);
+  if (op->nslices == 0) return 0;
+  const unsigned grid = (op->nslices + kWaves - 1) / kWaves;
+  k_spmv2<4><<<grid, 64 * kWaves, 0, S()>>>(op->off, op->col, op->val,
+                                            op->nrows, op->nslices, x0, x1, y0,
+                                            y1);
+  return check("spmv2 launch");
 }
 
 int acd_dev_gather(double* dest, const double* src, const uint32_t* p,
                    uint32_t n) {
   if (n == 0) return 0;
   k_gather<<<(n + 255) / 256, 256, 0, S()>>>(dest, src, p, n);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) {
-    acd_dev_record_error((int)e, "gather launch");
-    return (int)e;
-  }
-  return 0;
+  return check("gather launch");
 }
 
-int acd_dev_spmv_time(const struct acd_dev_matrix* m, int transpose, double* y,
-                      const double* x, int reps, double* OUT_ms) {
+// Mean milliseconds per launch of SpMV variant `variant` (internal: used
+// to choose the default; variant < 0 selects the default).
+int acd_dev_spmv_time_variant(const struct acd_dev_matrix* m, int transpose,
+                              double* y, const double* x, int reps,
+                              int variant, double* OUT_ms) {
+  if (variant < 0) variant = g_default_variant;
   hipEvent_t a, b;
   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
     return -1;
   const struct acd_sell* op = pick(m, transpose);
-  int rc = launch<1>(op, y, nullptr, x, nullptr);  // warm
+  int rc = launch1(op, y, x, variant);  // warm
   hipEventRecord(a, S());
-  for (int r = 0; r < reps && !rc; r++) rc = launch<1>(op, y, nullptr, x, nullptr);
+  for (int r = 0; r < reps && !rc; r++) rc = launch1(op, y, x, variant);
   hipEventRecord(b, S());
   hipEventSynchronize(b);
   float ms = 0;
@@ -138,6 +259,16 @@ int acd_dev_spmv_time(const struct acd_dev_matrix* m, int transpose, double* y,
   hipEventDestroy(a);
   hipEventDestroy(b);
   return rc;
+}
+
+int acd_dev_spmv_time(const struct acd_dev_matrix* m, int transpose, double* y,
+                      const double* x, int reps, double* OUT_ms) {
+  return acd_dev_spmv_time_variant(m, transpose, y, x, reps, -1, OUT_ms);
+}
+
+int acd_dev_spmv_set_default_variant(int variant) {
+  g_default_variant = variant;
+  return 0;
 }
 
 }  // extern "C"

@@ -69,6 +69,14 @@ double * alm_rhs(alm_t * alm) { return approx_rhs(alm->approx); }
 double * alm_lower(alm_t * alm) { return approx_lower(alm->approx); }
 double * alm_upper(alm_t * alm) { return approx_upper(alm->approx); }
 
+int acd_alm_reset(alm_t * alm)
+{
+        approx_t * a = alm->approx;
+        for (size_t i = 0; i < a->nrhs; i++) a->weight[i] = 1;
+        for (size_t i = 0; i < a->nvars; i++) a->linear[i] = 0;
+        return 0;
+}
+
 void acd_alm_default_options(struct acd_alm_options * o)
 {
         o->stop_violation = 1e-5;

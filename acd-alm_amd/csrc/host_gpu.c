@@ -42,3 +42,34 @@ uint64_t acd_matrix_stored_entries(const sparse_matrix_t * a, int transpose)
 {
         return transpose ? a->dev.at.nent : a->dev.a.nent;
 }
+
+int acd_spmv_variant_time(const sparse_matrix_t * a, int transpose,
+                          double * d_y, const double * d_x, int reps,
+                          int variant, double * OUT_ms)
+{
+        return acd_dev_spmv_time_variant(&a->dev, transpose, d_y, d_x, reps,
+                                         variant, OUT_ms);
+}
+
+int acd_spmv_set_variant(int variant)
+{
+        if (variant < 0 || variant >= ACD_SPMV_VARIANTS) return -1;
+        return acd_dev_spmv_set_default_variant(variant);
+}
+
+void acd_last_solve_profile(uint64_t * counts, double * us, int n)
+{
+        acd_dev_last_profile(counts, us, n);
+}
+
+const char * acd_pass_name(int p)
+{
+        static const char * names[] = {
+                "none", "init_x", "violation", "pass1", "pass2", "fixup",
+                "restart_copy", "checkpoint_x", "gradient_pg", "norm_g",
+                "finish", "permute_out", "spg_grad0", "spg_grad", "spg_step",
+                "alm_project", "alm_penalise", "permute_problem",
+                "step_sizes", "alm_violation", "alm_argmax", "alm_update"};
+        if (p < 0 || p >= (int)(sizeof names / sizeof names[0])) return "";
+        return names[p];
+}

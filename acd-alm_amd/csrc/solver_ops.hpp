@@ -1,0 +1,1139 @@
+// solver_ops.hpp -- the LP hot path of pkhuong/acd-alm as micro-ops.
+//
+// ALM outer loop (alm.c:247-322), APPROX inner loop (approx/approx.c:
+// 417-708) and spectral projected gradient (approx/mpg.inc:155-199,
+// approx/approx.c:517-573), on the row/column-permuted working copy of the
+// problem (approx/approx.c:84-88) stored as sliced ELL.
+//
+// The solver is split into
+//   * ops: one data-parallel pass each (an SpMV over A or A' fused with its
+//     elementwise epilogue, or an elementwise pass), producing up to eight
+//     partial sums/maxima per thread; and
+//   * a control state machine (Ctl) that consumes the reduced results of
+//     the last op, takes every decision of the reference (step acceptance,
+//     restarts, stopping tests, ALM updates) and names the next op.
+// For a large LP each op is one wide kernel and Ctl runs in a one-block
+// kernel after it (the kernel boundary is the global barrier, cheaper on
+// MI355X than an in-kernel grid barrier); for a batch of small LPs one
+// workgroup loops op -> workgroup reduction -> Ctl.  Ops visit index j
+// with a fixed lane in a fixed order and reductions combine in a fixed
+// order, so results are deterministic and identical for every launch.
+//
+// Pass structure of one APPROX iteration in the common (long step)
+// case: PASS1 over A' rows -- g2 = A'(y.viol) + c, [g = A'(z.viol) + c],
+// zp = clamp(z - g2 * itheta / v), model decrease, c'z, c'zp, fused with
+// the previous iteration's x <- (1-t)x + t z -- then PASS2 over A rows --
+// zp.viol = w (A zp - b) and its quadratic, plus the violation half of
+// the momentum update written speculatively to shadow buffers that are
+// adopted by swapping roles when the step is accepted.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include "acd_dev.h"
+
+namespace acd {
+
+constexpr int kRes = 8;  // result slots per op
+
+// n-side and m-side work vector roles.
+enum { NX = 0, NZ, NZP, NG, NG2, NPV, kNumVN };
+enum { MXV = 0, MYV, MZV, MZPV, MSX, MSY, kNumVM };
+
+enum Op : int32_t {
+  OP_NONE = 0,
+  OP_INIT_X,
+  OP_VIOL,
+  OP_PASS1,
+  OP_PASS2,
+  OP_FIXUP,
+  OP_RESTART_COPY,
+  OP_CKPT_X,
+  OP_GRAD_PG,
+  OP_NORM_G,
+  OP_FIN_A,
+  OP_PERMUTE_OUT,
+  OP_MPG_GRAD0,
+  OP_MPG_GRAD,
+  OP_MPG_ZUPD,
+  OP_ALM_LAMPROJ,
+  OP_PENALISE,
+  OP_PERMUTE_PROBLEM,
+  OP_STEP_SIZES,
+  OP_ALM_VIOL,
+  OP_ARGMAX,
+  OP_LAM_W
+};
+
+enum OpFlag : uint32_t {
+  F_APPROX = 1u << 0,  // INIT_X: approx variant (prev_x copy, c'x)
+  F_COPY2 = 1u << 1,   // VIOL: copy the violation into z.viol and y.viol
+  F_CX = 1u << 2,      // VIOL / PERMUTE_OUT: also c'x
+  F_TWO = 1u << 3,     // PASS1: gradient at z too
+  F_LONG = 1u << 4,    // PASS1: long step (model decrease)
+  F_PG = 1u << 5,      // PASS1: projected gradient at z
+  F_LAZY = 1u << 6,    // PASS1 / CKPT_X: apply pending x update
+  F_PCOPY = 1u << 7,   // PASS1 / FIN_A: prev_x <- x pending
+  F_SPEC = 1u << 8,    // PASS2: speculative (shadow) momentum update
+  F_NEEDG = 1u << 9,   // FIXUP: compute g at z
+  F_RESTEP = 1u << 10, // FIXUP: re-step from g2
+  F_UPD = 1u << 11,    // LAM_W: scale the weights
+  F_BUMP = 1u << 12    // LAM_W: weight[argmax] *= 10
+};
+
+// Control continuation points.
+enum Pc : int32_t {
+  PC_DONE = 0,
+  // APPROX
+  PC_A_INIT, PC_A_INIT1, PC_A_INIT2,
+  PC_I0, PC_I_SHORT, PC_I_LONG1, PC_I_LONG2, PC_I_STEPPED, PC_I_FIX,
+  PC_I_DD, PC_I_RESTARTED, PC_I_LIN2, PC_I_COMMIT, PC_I_DRIVER,
+  PC_I_CK1, PC_I_CK2, PC_I_CK3, PC_I_PRINT, PC_I_PRINT2, PC_I_NEXT,
+  PC_F0, PC_F1, PC_F2, PC_F3,
+  // SPG
+  PC_M0, PC_M1, PC_M2, PC_M3, PC_MI, PC_MI2, PC_MI3, PC_MI5,
+  PC_MF1, PC_MF2,
+  // ALM
+  PC_L0, PC_L1, PC_LB0, PC_LB1, PC_LB2, PC_LB3,
+  PC_LE0, PC_LE1, PC_LE2, PC_LE3, PC_LE4,
+  // approx_update
+  PC_U0, PC_U1, PC_U2
+};
+
+struct Sell {
+  const uint64_t* off;
+  const uint32_t* col;
+  const double* val;
+  uint32_t nrows, nslices;
+};
+
+struct State {
+  // control
+  int32_t pc, mode, log, error;
+  int32_t ret, reason, center_x, restart_flag;
+  int32_t lazy_x, prevx_pending, violp_zp, events_full;
+  // current op
+  int32_t op, op_r0, op_r1, pad0;
+  uint32_t op_flags, op_maxmask, op_u, pad1;
+  double op_a, op_b, op_c;
+  double* pn[kNumVN];
+  double* pm[kNumVM];
+  // APPROX
+  uint64_t niter, period, i, k;
+  double max_pg, max_value, min_delta, offset;
+  double theta, step_length, lazy_theta;
+  double quad_x, quad_z, quad_zp;
+  double value, pg, delta;
+  double diag[5];
+  // values carried between continuations of one iteration
+  double t_expected, t_cz, t_czp, t_dd, t_pg2, t_len, t_x0, t_x1, t_x2;
+  int32_t t_descent, t_got, t_restep, t_have_dd, t_restarted, t_safe;
+  // ALM
+  uint64_t outer, outer_niter;
+  double precision, prev_norm, max_viol, alm_pg, cx, norm, w1_pre;
+  double stop_violation, stop_pg, precision_floor, inner_min_delta;
+  double alm_diag[3];
+  int32_t alm_ret, inner_reason, upd, bump;
+  double scale, argmax;
+  // bookkeeping
+  uint64_t inner_total, events, ops;
+  uint64_t op_count[32];        // launches per op
+  uint64_t op_ticks[32];        // wall-clock ticks per op (step + control)
+  uint64_t t_last;              // wall clock when the last op was issued
+};
+
+struct Args {
+  uint32_t m, n;
+  Sell A, AT, A0, AT0;          // working copy and caller's matrix
+  const uint32_t* row_idx;      // working copy permutations
+  const uint32_t* col_idx;
+  const uint32_t* col_ridx;
+  const uint32_t* beta;         // entries per row of the working copy
+  // caller-order vectors
+  double *rhs, *weight, *linc, *lower, *upper, *alm_c, *lam_lo, *lam_hi;
+  double *x_io, *lam, *r;
+  // working-copy vectors
+  double *p_rhs, *p_w, *p_c, *p_l, *p_u, *p_v, *p_iv;
+  State* st;
+  double* partials;             // [blocks][kRes]
+  acd_event* ev;
+  uint32_t ev_cap;
+};
+
+// SSE2 maxpd(a, b) / minpd(a, b) of the reference's projections
+// (approx/step_project.inc:18-19, :53-54, approx/approx.c:272-273).
+__device__ __forceinline__ double maxpd(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double minpd(double a, double b) { return a < b ? a : b; }
+
+__device__ __forceinline__ double combine(double a, double b, bool is_max) {
+  return is_max ? fmax(a, b) : a + b;
+}
+
+__device__ __forceinline__ double wave_reduce(double v, bool is_max) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = combine(v, __shfl_xor(v, o, 64), is_max);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Ops.  G supplies lane, gwave (this wave's index among the group's waves)
+// and nwaves; slice s of 64 rows goes to wave s mod nwaves.
+template <class G>
+struct Ops {
+  const G& g;
+  const Args& a;
+  const State& s;
+  double p[kRes];
+
+  __device__ __forceinline__ Ops(const G& g_, const Args& a_, const State& s_)
+      : g(g_), a(a_), s(s_) {
+#pragma unroll
+    for (int k = 0; k < kRes; k++) p[k] = 0.0;
+  }
+
+  __device__ __forceinline__ double* VN(int r) const { return s.pn[r]; }
+  __device__ __forceinline__ double* VM(int r) const { return s.pm[r]; }
+
+  // Row r of M summed lane-sequentially in ascending column order from
+  // +0.0 with separate multiply and add (the reference's per-row order,
+  // spmv/spmv_block.c:160-254); epi(row, acc0, acc1).
+  template <int NRHS, class E>
+  __device__ __forceinline__ void spmv_rows(const Sell& M, const double* x0,
+                                            const double* x1, E&& epi) {
+    for (uint32_t sl = g.gwave; sl < M.nslices; sl += g.nwaves) {
+      const uint64_t base = M.off[sl];
+      const uint32_t len = (uint32_t)((M.off[sl + 1] - base) >> 6);
+      const double* v = M.val + base + g.lane;
+      const uint32_t* c = M.col + base + g.lane;
+      double a0 = 0.0, a1 = 0.0;
+      uint32_t k = 0;
+      for (; k + 4 <= len; k += 4, v += 256, c += 256) {
+        const double v0 = v[0], v1 = v[64], v2 = v[128], v3 = v[192];
+        const uint32_t c0 = c[0], c1 = c[64], c2 = c[128], c3 = c[192];
+        const double p0 = x0[c0], p1 = x0[c1], p2 = x0[c2], p3 = x0[c3];
+        a0 = a0 + v0 * p0;
+        a0 = a0 + v1 * p1;
+        a0 = a0 + v2 * p2;
+        a0 = a0 + v3 * p3;
+        if (NRHS == 2) {
+          const double q0 = x1[c0], q1 = x1[c1], q2 = x1[c2], q3 = x1[c3];
+          a1 = a1 + v0 * q0;
+          a1 = a1 + v1 * q1;
+          a1 = a1 + v2 * q2;
+          a1 = a1 + v3 * q3;
+        }
+      }
+      for (; k < len; k++, v += 64, c += 64) {
+        const double vv = *v;
+        const uint32_t cc = *c;
+        a0 = a0 + vv * x0[cc];
+        if (NRHS == 2) a1 = a1 + vv * x1[cc];
+      }
+      const uint32_t r = sl * 64 + g.lane;
+      if (r < M.nrows) epi(r, a0, a1);
+    }
+  }
+
+  template <class F>
+  __device__ __forceinline__ void each(uint32_t n, F&& f) {
+    const uint32_t ns = (n + 63) >> 6;
+    for (uint32_t sl = g.gwave; sl < ns; sl += g.nwaves) {
+      const uint32_t r = sl * 64 + g.lane;
+      if (r < n) f(r);
+    }
+  }
+
+  __device__ __forceinline__ void run() {
+    const uint32_t fl = s.op_flags;
+    switch (s.op) {
+      case OP_INIT_X: {  // approx/approx.c:591-599, :534-537
+        double* X = VN(NX);
+        double* Z = VN(NZ);
+        double* PV = VN(NPV);
+        each(a.n, [&](uint32_t j) {
+          double xj = a.x_io[a.col_idx[j]];
+          const double lo = maxpd(a.p_l[j], xj);
+          xj = minpd(lo, a.p_u[j]);
+          X[j] = xj;
+          Z[j] = xj;
+          if (fl & F_APPROX) {
+            PV[j] = xj;
+            p[0] += a.p_c[j] * xj;
+          }
+        });
+        break;
+      }
+      case OP_VIOL: {  // approx/gradient_value.inc:15-81
+        const double* X = VN(s.op_r0);
+        double* V = VM(s.op_r1);
+        double* c1 = VM(MZV);
+        double* c2 = VM(MYV);
+        const bool copy2 = fl & F_COPY2;
+        spmv_rows<1>(a.A, X, nullptr, [&](uint32_t i, double ax, double) {
+          const double v = ax - a.p_rhs[i];
+          const double sv = a.p_w[i] * v;
+          V[i] = sv;
+          if (copy2) {
+            c1[i] = sv;
+            c2[i] = sv;
+          }
+          p[0] += sv * v;
+        });
+        if (fl & F_CX) each(a.n, [&](uint32_t j) { p[1] += a.p_c[j] * X[j]; });
+        break;
+      }
+      case OP_PASS1: {  // step_project.inc:179-236, :320-375, :452-508
+        double* Zp = VN(NZP);
+        const double* Z = VN(NZ);
+        double* G1 = VN(NG);
+        double* G2 = VN(NG2);
+        double* X = VN(NX);
+        double* PV = VN(NPV);
+        const bool two = fl & F_TWO, lng = fl & F_LONG, want_pg = fl & F_PG;
+        const bool lazy = fl & F_LAZY, pcopy = fl & F_PCOPY;
+        const double itheta = s.op_a, theta2 = s.op_b;
+        const double lt = s.op_c, ls = 1 - s.op_c;
+        auto epi = [&](uint32_t j, double gy, double gz) {
+          const double cj = a.p_c[j];
+          const double zj = Z[j];
+          if (pcopy || lazy) {
+            const double xj = X[j];
+            if (pcopy) PV[j] = xj;
+            if (lazy) X[j] = ls * xj + lt * zj;
+          }
+          const double g2 = gy + cj;
+          G2[j] = g2;
+          const double stp = itheta * a.p_iv[j];
+          double t = zj - g2 * stp;
+          t = maxpd(a.p_l[j], t);
+          t = minpd(a.p_u[j], t);
+          Zp[j] = t;
+          p[4] = maxpd(p[4], fabs(t));
+          p[2] += cj * zj;
+          p[3] += cj * t;
+          if (lng) {
+            const double d = t - zj;
+            p[0] += g2 * d;
+            p[1] += d * d * theta2 * a.p_v[j];
+          }
+          if (two) {
+            const double gg = gz + cj;
+            G1[j] = gg;
+            p[5] += gg * (t - zj);
+            if (want_pg) {
+              double xp = zj - gg;
+              xp = maxpd(a.p_l[j], xp);
+              xp = minpd(a.p_u[j], xp);
+              const double d = zj - xp;
+              p[6] += d * d;
+            }
+          }
+        };
+        if (two)
+          spmv_rows<2>(a.AT, VM(MYV), VM(MZV), epi);
+        else
+          spmv_rows<1>(a.AT, VM(MYV), nullptr, epi);
+        break;
+      }
+      case OP_PASS2: {  // compute_violation(zp) + approx/vector.inc:249-295
+        const double* Zp = VN(NZP);
+        double* ZPV = VM(MZPV);
+        double* XV = VM(MXV);
+        double* YV = VM(MYV);
+        double* SX = VM(MSX);
+        double* SY = VM(MSY);
+        const bool spec = fl & F_SPEC;
+        const double th = s.op_a, thn = s.op_b;
+        const double sc = 1 - th, nsc = 1 - thn;
+        spmv_rows<1>(a.A, Zp, nullptr, [&](uint32_t i, double ax, double) {
+          const double v = ax - a.p_rhs[i];
+          const double sv = a.p_w[i] * v;
+          ZPV[i] = sv;
+          p[0] += sv * v;
+          const double xv = sc * XV[i] + th * sv;
+          const double yv = nsc * xv + thn * sv;
+          if (spec) {
+            SX[i] = xv;
+            SY[i] = yv;
+          } else {
+            XV[i] = xv;
+            YV[i] = yv;
+          }
+        });
+        break;
+      }
+      case OP_FIXUP: {  // approx/approx.c:383-389, :445-450
+        double* Zp = VN(NZP);
+        const double* Z = VN(NZ);
+        double* G1 = VN(NG);
+        const double* G2 = VN(NG2);
+        const bool needg = fl & F_NEEDG, restep = fl & F_RESTEP;
+        const double itheta = s.op_a;
+        auto body = [&](uint32_t j, double gz) {
+          const double zj = Z[j];
+          double gg;
+          if (needg) {
+            gg = gz + a.p_c[j];
+            G1[j] = gg;
+          } else {
+            gg = G1[j];
+          }
+          double t;
+          if (restep) {
+            const double stp = itheta * a.p_iv[j];
+            t = zj - G2[j] * stp;
+            t = maxpd(a.p_l[j], t);
+            t = minpd(a.p_u[j], t);
+            Zp[j] = t;
+            p[1] = maxpd(p[1], fabs(t));
+          } else {
+            t = Zp[j];
+          }
+          p[0] += gg * (t - zj);
+        };
+        if (needg)
+          spmv_rows<1>(a.AT, VM(MZV), nullptr,
+                       [&](uint32_t j, double gz, double) { body(j, gz); });
+        else
+          each(a.n, [&](uint32_t j) { body(j, 0.0); });
+        break;
+      }
+      case OP_RESTART_COPY: {  // approx/approx.c:452-453
+        double* X = VN(NX);
+        const double* Z = VN(NZ);
+        double* XV = VM(MXV);
+        double* YV = VM(MYV);
+        const double* ZV = VM(MZV);
+        each(a.n, [&](uint32_t j) { X[j] = Z[j]; });
+        each(a.m, [&](uint32_t r) {
+          const double v = ZV[r];
+          XV[r] = v;
+          YV[r] = v;
+        });
+        break;
+      }
+      case OP_CKPT_X: {  // approx/approx.c:641-651
+        double* X = VN(NX);
+        const double* Z = VN(NZ);
+        const double* PV = VN(NPV);
+        const bool lazy = fl & F_LAZY;
+        const double lt = s.op_c, ls = 1 - s.op_c;
+        each(a.n, [&](uint32_t j) {
+          double xj = X[j];
+          if (lazy) {
+            xj = ls * xj + lt * Z[j];
+            X[j] = xj;
+          }
+          p[0] += a.p_c[j] * xj;
+          const double d = PV[j] - xj;
+          p[1] += d * d;
+          p[2] += xj * xj;
+        });
+        break;
+      }
+      case OP_GRAD_PG: {  // gradient_value.inc:158-181, approx.c:258-278
+        const double* V = VM(s.op_r1);
+        const double* X = VN(s.op_r0);
+        double* Gout = VN(NG);
+        spmv_rows<1>(a.AT, V, nullptr, [&](uint32_t j, double gx, double) {
+          const double gg = gx + a.p_c[j];
+          Gout[j] = gg;
+          const double xj = X[j];
+          double xp = xj - gg;
+          xp = maxpd(a.p_l[j], xp);
+          xp = minpd(a.p_u[j], xp);
+          const double d = xj - xp;
+          p[0] += d * d;
+          p[1] += gg * gg;
+        });
+        break;
+      }
+      case OP_NORM_G: {
+        const double* Gv = VN(NG);
+        each(a.n, [&](uint32_t j) { p[0] += Gv[j] * Gv[j]; });
+        break;
+      }
+      case OP_FIN_A: {  // approx/approx.c:681-683
+        const double* C = VN(s.op_r0);
+        const double* X = VN(NX);
+        const double* PV = VN(NPV);
+        const bool pcopy = fl & F_PCOPY;
+        each(a.n, [&](uint32_t j) {
+          const double cj = C[j];
+          const double pv = pcopy ? X[j] : PV[j];
+          const double d = pv - cj;
+          p[0] += d * d;
+          p[1] += cj * cj;
+          p[2] += a.p_c[j] * cj;
+        });
+        break;
+      }
+      case OP_PERMUTE_OUT: {  // approx/approx.c:693-694
+        const double* C = VN(s.op_r0);
+        each(a.n, [&](uint32_t i) { a.x_io[i] = C[a.col_ridx[i]]; });
+        if (fl & F_CX) each(a.n, [&](uint32_t j) { p[0] += a.p_c[j] * C[j]; });
+        break;
+      }
+      case OP_MPG_GRAD0: {  // approx/approx.c:538-539
+        double* G1 = VN(NG);
+        double* G2 = VN(NG2);
+        spmv_rows<1>(a.AT, VM(MXV), nullptr, [&](uint32_t j, double gx, double) {
+          const double gg = gx + a.p_c[j];
+          G1[j] = gg;
+          G2[j] = gg;
+        });
+        break;
+      }
+      case OP_MPG_GRAD: {  // approx/mpg.inc:161-174
+        const double* X = VN(NX);
+        const double* Z = VN(NZ);
+        double* G1 = VN(NG);
+        const double* G2 = VN(NG2);
+        spmv_rows<1>(a.AT, VM(MXV), nullptr, [&](uint32_t j, double gx, double) {
+          const double gg = gx + a.p_c[j];
+          G1[j] = gg;
+          const double xj = X[j];
+          double xp = xj - gg;
+          xp = maxpd(a.p_l[j], xp);
+          xp = minpd(a.p_u[j], xp);
+          const double d = xj - xp;
+          p[0] += d * d;
+          const double yj = xj - Z[j];
+          const double uj = gg - G2[j];
+          p[1] += yj * yj;
+          p[2] += yj * uj;
+        });
+        break;
+      }
+      case OP_MPG_ZUPD: {  // approx/mpg.inc:180-182
+        const double* X = VN(NX);
+        double* Z = VN(NZ);
+        const double* G1 = VN(NG);
+        const double st = s.op_a;
+        each(a.n, [&](uint32_t j) {
+          const double t = X[j] - st * G1[j];
+          const double lo = maxpd(a.p_l[j], t);
+          Z[j] = minpd(lo, a.p_u[j]);
+        });
+        break;
+      }
+      case OP_ALM_LAMPROJ: {  // alm.c:171-180, :314
+        each(a.m, [&](uint32_t i) {
+          a.lam[i] = fmax(a.lam_lo[i], fmin(a.lam[i], a.lam_hi[i]));
+        });
+        each(a.n, [&](uint32_t j) { p[0] += a.x_io[j] * a.alm_c[j]; });
+        break;
+      }
+      case OP_PENALISE: {  // alm.c:106-119
+        spmv_rows<1>(a.AT0, a.lam, nullptr, [&](uint32_t j, double atl, double) {
+          a.linc[j] = a.alm_c[j] - atl;
+        });
+        each(a.m, [&](uint32_t i) { p[0] += a.lam[i] * a.rhs[i]; });
+        break;
+      }
+      case OP_PERMUTE_PROBLEM: {  // approx/approx.c:192-211
+        double* wb = VM(MSX);
+        each(a.m, [&](uint32_t i) {
+          const uint32_t src = a.row_idx[i];
+          a.p_rhs[i] = a.rhs[src];
+          const double w = a.weight[src];
+          a.p_w[i] = w;
+          wb[i] = w * (double)a.beta[i];
+        });
+        each(a.n, [&](uint32_t j) {
+          const uint32_t src = a.col_idx[j];
+          a.p_c[j] = a.linc[src];
+          a.p_l[j] = a.lower[src];
+          a.p_u[j] = a.upper[src];
+        });
+        break;
+      }
+      case OP_STEP_SIZES: {  // approx/approx.c:135-186
+        const double* wb = VM(MSX);
+        const Sell& M = a.AT;
+        for (uint32_t sl = g.gwave; sl < M.nslices; sl += g.nwaves) {
+          const uint64_t base = M.off[sl];
+          const uint32_t len = (uint32_t)((M.off[sl + 1] - base) >> 6);
+          double acc = 0.0;
+          for (uint32_t k = 0; k < len; k++) {
+            const uint64_t e = base + (uint64_t)k * 64 + g.lane;
+            const double val = M.val[e];
+            acc = acc + (wb[M.col[e]] * val) * val;
+          }
+          const uint32_t j = sl * 64 + g.lane;
+          if (j < M.nrows) {
+            a.p_v[j] = acc;
+            a.p_iv[j] = (acc == 0 && a.p_c[j] == 0) ? 1.0 : 1.0 / acc;
+          }
+        }
+        break;
+      }
+      case OP_ALM_VIOL: {  // alm.c:121-132, :134-169
+        p[1] = -HUGE_VAL;
+        spmv_rows<1>(a.A0, a.x_io, nullptr, [&](uint32_t i, double ax, double) {
+          const double ri = ax - a.rhs[i];
+          a.r[i] = ri;
+          p[0] += ri * ri;
+          p[1] = fmax(p[1], fabs(ri));
+          p[2] += fabs(a.weight[i]);
+        });
+        each(a.n, [&](uint32_t j) { p[3] += a.x_io[j] * a.alm_c[j]; });
+        break;
+      }
+      case OP_ARGMAX: {  // alm.c:155-166: first index of the maximum
+        const double mv = s.op_a;
+        p[0] = -HUGE_VAL;
+        each(a.m, [&](uint32_t i) {
+          if (fabs(a.r[i]) == mv) p[0] = fmax(p[0], -(double)i);
+        });
+        break;
+      }
+      case OP_LAM_W: {  // alm.c:182-212
+        const bool upd = fl & F_UPD, bump = fl & F_BUMP;
+        const double scale = s.op_a;
+        const uint32_t am = s.op_u;
+        each(a.m, [&](uint32_t i) {
+          const double li = a.lam[i] - a.weight[i] * a.r[i];
+          a.lam[i] = fmax(a.lam_lo[i], fmin(li, a.lam_hi[i]));
+          if (upd) {
+            double w = a.weight[i] * scale;
+            if (bump && i == am) w *= 10;
+            a.weight[i] = w;
+          }
+          p[0] += fabs(a.weight[i]);
+        });
+        break;
+      }
+      default:
+        break;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Control: consumes the reduced results of the op just run, advances.
+struct Ctl {
+  State& s;
+  const Args& a;
+  const double* res;
+
+  __device__ void emit(int32_t type, uint64_t k, double a0 = 0, double a1 = 0,
+                       double a2 = 0, double a3 = 0, double a4 = 0,
+                       double a5 = 0) {
+    if (s.events < a.ev_cap) {
+      acd_event* e = &a.ev[s.events];
+      e->type = type;
+      e->pad = 0;
+      e->k = k;
+      e->a[0] = a0; e->a[1] = a1; e->a[2] = a2;
+      e->a[3] = a3; e->a[4] = a4; e->a[5] = a5;
+      s.events++;
+    }
+  }
+
+  __device__ void issue(int32_t op, uint32_t flags, int32_t next_pc,
+                        uint32_t maxmask = 0, double oa = 0, double ob = 0,
+                        double oc = 0) {
+    s.op = op;
+    s.op_flags = flags;
+    s.op_maxmask = maxmask;
+    s.op_a = oa;
+    s.op_b = ob;
+    s.op_c = oc;
+    s.pc = next_pc;
+    s.ops++;
+  }
+
+  __device__ void fail(int32_t code) {
+    s.error = code;
+    s.pc = PC_DONE;
+    s.op = OP_NONE;
+  }
+
+  template <class T>
+  __device__ static void swap_(T& x, T& y) {
+    T t = x;
+    x = y;
+    y = t;
+  }
+
+  __device__ void stop(int32_t reason) {
+    s.reason = reason;
+    s.pc = PC_F0;
+  }
+
+  // Runs continuations until one issues an op (returns) or the solve ends.
+  __device__ void advance() {
+    s.op = OP_NONE;
+    for (int guard = 0; guard < 256 && s.op == OP_NONE && s.pc != PC_DONE;
+         guard++) {
+      step();
+    }
+  }
+
+  __device__ void step() {
+    switch (s.pc) {
+      // ------------------------------------------------------------ APPROX
+      case PC_A_INIT:  // approx/approx.c:588-607
+        issue(OP_INIT_X, F_APPROX, PC_A_INIT1);
+        break;
+      case PC_A_INIT1:
+        s.cx = res[0];
+        s.op_r0 = NX;
+        s.op_r1 = MXV;
+        issue(OP_VIOL, F_COPY2, PC_A_INIT2);
+        break;
+      case PC_A_INIT2: {
+        const double q = .5 * res[0];
+        s.quad_x = s.quad_z = q;
+        s.violp_zp = 0;
+        s.value = s.cx + q + s.offset;
+        s.i = 0;
+        s.k = 0;
+        s.theta = 1;
+        s.step_length = 1;
+        s.lazy_x = 0;
+        s.prevx_pending = 0;
+        s.center_x = 1;
+        s.reason = 0;
+        s.restart_flag = 0;
+        s.pg = HUGE_VAL;
+        s.delta = HUGE_VAL;
+        emit(ACD_EV_APPROX_BEGIN, 0);
+        s.pc = s.niter > 0 ? PC_I0 : PC_F0;
+        break;
+      }
+      case PC_I0: {  // approx/approx.c:608-616, :417-437
+        if (s.i >= s.niter) {
+          s.pc = PC_F0;
+          break;
+        }
+        if (s.events + 8 > a.ev_cap) {  // let the host drain the log
+          s.events_full = 1;
+          s.op = OP_NONE;
+          s.pc = PC_I0;
+          return;
+        }
+        const bool pg_req = (s.i == 0) || ((s.i + 1) % 10 == 0);
+        const double len = s.step_length;
+        uint32_t fl = (pg_req ? F_PG : 0) | (s.lazy_x ? F_LAZY : 0) |
+                      (s.prevx_pending ? F_PCOPY : 0);
+        s.t_len = len;
+        if (len <= 1) {
+          issue(OP_PASS1, fl | F_TWO, PC_I_SHORT, 1u << 4,
+                (1 - 1e-6) / s.theta, 0.0, s.lazy_theta);
+        } else {
+          const bool got = pg_req || (len <= 1 + 1e-6);
+          const bool safe = len <= 1 + 1e-6;
+          const double L = safe ? 1.0 : len;
+          s.t_got = got;
+          s.t_safe = safe;
+          issue(OP_PASS1, fl | F_LONG | (got ? F_TWO : 0), PC_I_LONG1, 1u << 4,
+                (L - 1e-6) / s.theta, s.theta / L, s.lazy_theta);
+        }
+        s.lazy_x = 0;
+        s.prevx_pending = 0;
+        break;
+      }
+      case PC_I_SHORT: {  // approx/approx.c:394-415
+        if (!(res[4] < HUGE_VAL)) return fail(ACD_ERR_UNBOUNDED);
+        s.step_length = s.t_len * 1.01;
+        if (s.step_length > 1) s.step_length = 1 + 1e-6;
+        s.t_descent = 0;
+        s.t_got = 1;
+        s.t_restep = 0;
+        s.t_dd = res[5];
+        s.t_have_dd = 1;
+        s.violp_zp = 0;
+        s.t_cz = res[2];
+        s.t_pg2 = res[6];
+        s.pc = PC_I_STEPPED;
+        break;
+      }
+      case PC_I_LONG1: {  // approx/approx.c:343-376
+        s.t_expected = res[0] + .5 * res[1];
+        s.t_cz = res[2];
+        s.t_czp = res[3];
+        s.t_dd = res[5];
+        s.t_pg2 = res[6];
+        issue(OP_PASS2, F_SPEC, PC_I_LONG2, 0, s.theta,
+              2.0 / (double)(s.k + 1 + 2));
+        break;
+      }
+      case PC_I_LONG2: {  // approx/approx.c:374-391
+        const double qzp = .5 * res[0];
+        s.quad_zp = qzp;
+        s.violp_zp = 1;
+        const double L = s.t_safe ? 1.0 : s.t_len;
+        const double initial = s.t_cz + s.quad_z;
+        const double now = s.t_czp + qzp;
+        if (!(s.t_expected <= 0)) return fail(ACD_ERR_ESTIMATE);
+        s.t_restep = 0;
+        if (now <= initial + s.t_expected) {
+          s.step_length = L * 1.01;
+          s.t_descent = 1;
+        } else {
+          s.step_length = .9 * L;
+          s.t_descent = 0;
+          s.t_restep = !s.t_safe;
+        }
+        s.t_have_dd = s.t_got && !s.t_restep;
+        s.pc = PC_I_STEPPED;
+        break;
+      }
+      case PC_I_STEPPED: {  // approx/approx.c:439-450
+        const bool pg_req = (s.i == 0) || ((s.i + 1) % 10 == 0);
+        s.pg = pg_req ? sqrt(s.t_pg2) : HUGE_VAL;
+        s.delta = HUGE_VAL;
+        s.t_restarted = 0;
+        if (!s.t_descent && !s.t_have_dd) {
+          issue(OP_FIXUP, (s.t_got ? 0 : F_NEEDG) | (s.t_restep ? F_RESTEP : 0),
+                PC_I_FIX, 2u, (1 - 1e-6) / s.theta);
+        } else {
+          s.pc = PC_I_DD;
+        }
+        break;
+      }
+      case PC_I_FIX:
+        s.t_dd = res[0];
+        if (s.t_restep) {
+          if (!(res[1] < HUGE_VAL)) return fail(ACD_ERR_UNBOUNDED);
+          s.violp_zp = 0;
+        }
+        s.pc = PC_I_DD;
+        break;
+      case PC_I_DD:
+        if (!s.t_descent && s.t_dd > 0) {  // oscillation: restart
+          issue(OP_RESTART_COPY, 0, PC_I_RESTARTED);
+        } else if (!s.violp_zp) {
+          issue(OP_PASS2, 0, PC_I_LIN2, 0, s.theta,
+                2.0 / (double)(s.k + 1 + 2));
+        } else {
+          swap_(s.pm[MXV], s.pm[MSX]);
+          swap_(s.pm[MYV], s.pm[MSY]);
+          s.pc = PC_I_COMMIT;
+        }
+        break;
+      case PC_I_RESTARTED:  // approx/approx.c:452-457
+        s.quad_x = s.quad_z;
+        s.lazy_x = 0;
+        s.k = 0;
+        s.theta = 1;
+        s.center_x = 1;
+        s.t_restarted = 1;
+        s.pc = PC_I_DRIVER;
+        break;
+      case PC_I_LIN2:
+        s.quad_zp = .5 * res[0];
+        s.violp_zp = 1;
+        s.pc = PC_I_COMMIT;
+        break;
+      case PC_I_COMMIT: {  // approx/approx.c:461-477
+        const double th = s.theta;
+        s.k += 1;
+        s.theta = 2.0 / (double)(s.k + 2);
+        s.lazy_x = 1;
+        s.lazy_theta = th;
+        swap_(s.pn[NZ], s.pn[NZP]);
+        swap_(s.pm[MZV], s.pm[MZPV]);
+        swap_(s.quad_z, s.quad_zp);
+        s.center_x = 0;
+        s.pc = PC_I_DRIVER;
+        break;
+      }
+      case PC_I_DRIVER: {  // approx/approx.c:617-640
+        s.inner_total++;
+        if (s.t_restarted) {
+          if (s.log) emit(ACD_EV_RESTART, s.i);
+          s.restart_flag = 1;
+        }
+        const double value =
+            s.t_cz + (s.center_x ? s.quad_x : s.quad_zp) + s.offset;
+        s.value = value;
+        if (value < s.max_value) return stop(1);
+        if (s.pg < s.max_pg) return stop(2);
+        if ((s.i + 1) % 100 == 0) {
+          issue(OP_CKPT_X, s.lazy_x ? F_LAZY : 0, PC_I_CK1, 0, 0, 0,
+                s.lazy_theta);
+        } else {
+          s.pc = PC_I_PRINT;
+        }
+        break;
+      }
+      case PC_I_CK1:  // approx/approx.c:641-651
+        s.t_x0 = res[0];
+        s.t_x1 = res[1];
+        s.t_x2 = res[2];
+        s.lazy_x = 0;
+        s.op_r0 = NX;
+        s.op_r1 = MXV;
+        issue(OP_VIOL, 0, PC_I_CK2);
+        break;
+      case PC_I_CK2:
+        s.quad_x = .5 * res[0];
+        s.value = s.t_x0 + s.quad_x + s.offset;
+        s.op_r0 = NX;
+        s.op_r1 = MXV;
+        issue(OP_GRAD_PG, 0, PC_I_CK3);
+        break;
+      case PC_I_CK3:  // approx/approx.c:647-665
+        s.pg = sqrt(res[0]);
+        s.delta = sqrt(s.t_x1) / (sqrt(s.t_x2) + 1e-10);
+        s.center_x = 1;
+        if (s.value < s.max_value) return stop(1);
+        if (s.pg < s.max_pg) return stop(2);
+        if (s.delta < s.min_delta) return stop(3);
+        s.prevx_pending = 1;
+        s.pc = PC_I_PRINT;
+        break;
+      case PC_I_PRINT:  // approx/approx.c:666-674
+        if (s.i == 0 || (s.period && (s.i + 1) % s.period == 0)) {
+          if (s.restart_flag) {
+            emit(ACD_EV_NEWLINE, s.i);
+            s.restart_flag = 0;
+          }
+          if (s.log) {
+            issue(OP_NORM_G, 0, PC_I_PRINT2);
+            break;
+          }
+        }
+        s.pc = PC_I_NEXT;
+        break;
+      case PC_I_PRINT2:
+        emit(ACD_EV_PRINT, s.i + 1, s.value, sqrt(res[0]), s.pg, s.step_length,
+             s.delta);
+        s.pc = PC_I_NEXT;
+        break;
+      case PC_I_NEXT:
+        s.i += 1;
+        s.pc = PC_I0;
+        break;
+      case PC_F0:  // approx/approx.c:676-683
+        s.op_r0 = s.center_x ? NX : NZP;
+        issue(OP_FIN_A, s.prevx_pending ? F_PCOPY : 0, PC_F1);
+        break;
+      case PC_F1:
+        s.delta = sqrt(res[0]) / (sqrt(res[1]) + 1e-10);
+        s.value = res[2] + (s.center_x ? s.quad_x : s.quad_zp) + s.offset;
+        s.op_r0 = s.center_x ? NX : NZP;
+        s.op_r1 = s.center_x ? MXV : MZPV;
+        issue(OP_GRAD_PG, 0, PC_F2);
+        break;
+      case PC_F2:
+        s.diag[1] = sqrt(res[1]);
+        s.diag[2] = sqrt(res[0]);
+        s.op_r0 = s.center_x ? NX : NZP;
+        issue(OP_PERMUTE_OUT, 0, PC_F3);
+        break;
+      case PC_F3:  // approx/approx.c:684-702
+        if (s.restart_flag) {
+          emit(ACD_EV_NEWLINE, s.i);
+          s.restart_flag = 0;
+        }
+        s.diag[0] = s.value;
+        s.diag[3] = s.delta;
+        s.diag[4] = (double)(s.i + 1);
+        if (s.log)
+          emit(ACD_EV_PRINT, s.i + 1, s.value, s.diag[1], s.diag[2],
+               s.step_length, s.delta);
+        s.ret = s.reason;
+        s.pc = (s.mode == ACD_MODE_ALM) ? PC_LE0 : PC_DONE;
+        break;
+      // ------------------------------------------------------------ SPG
+      case PC_M0:  // approx/approx.c:534-541
+        issue(OP_INIT_X, 0, PC_M1);
+        break;
+      case PC_M1:
+        s.op_r0 = NX;
+        s.op_r1 = MXV;
+        issue(OP_VIOL, 0, PC_M2);
+        break;
+      case PC_M2:
+        s.quad_x = .5 * res[0];
+        issue(OP_MPG_GRAD0, 0, PC_M3);
+        break;
+      case PC_M3:
+        s.i = 0;
+        s.reason = 0;
+        s.pc = PC_MI;
+        break;
+      case PC_MI:  // approx/approx.c:543-545; mpg.inc:155-199
+        if (s.i >= s.niter || s.reason) {
+          s.op_r0 = NX;
+          issue(OP_PERMUTE_OUT, F_CX, PC_MF1);
+          break;
+        }
+        if (s.events + 8 > a.ev_cap) {
+          s.events_full = 1;
+          s.op = OP_NONE;
+          return;
+        }
+        issue(OP_MPG_GRAD, 0, PC_MI2);
+        break;
+      case PC_MI2: {
+        s.pg = sqrt(res[0]);
+        double st = res[1] / res[2];
+        if (isnan(st)) st = 1;
+        if (!(st > 1e8)) {
+          issue(OP_MPG_ZUPD, 0, PC_MI3, 0, st);
+        } else {
+          s.op_r0 = NX;
+          s.op_r1 = MXV;
+          issue(OP_VIOL, F_CX, PC_MI5);
+        }
+        break;
+      }
+      case PC_MI3:
+        swap_(s.pn[NZ], s.pn[NX]);
+        swap_(s.pn[NG2], s.pn[NG]);
+        s.op_r0 = NX;
+        s.op_r1 = MXV;
+        issue(OP_VIOL, F_CX, PC_MI5);
+        break;
+      case PC_MI5: {  // approx/approx.c:546-553
+        s.quad_x = .5 * res[0];
+        const double value = s.offset + (res[1] + s.quad_x);
+        if (s.pg < s.max_pg) s.reason = 2;
+        if (value < s.max_value) s.reason = 1;
+        if (s.log &&
+            ((s.period && (s.i % s.period == 0)) || (s.i == 0) || s.reason))
+          emit(ACD_EV_MPG, s.i + 1, s.pg, value);
+        s.value = value;
+        s.inner_total++;
+        s.i += 1;
+        s.pc = PC_MI;
+        break;
+      }
+      case PC_MF1:  // approx/approx.c:559-569
+        s.cx = res[0];
+        s.op_r0 = NX;
+        s.op_r1 = MXV;
+        issue(OP_GRAD_PG, 0, PC_MF2);
+        break;
+      case PC_MF2:
+        s.diag[0] = s.offset + (s.cx + s.quad_x);
+        s.diag[1] = sqrt(res[1]);
+        s.diag[2] = sqrt(res[0]);
+        s.diag[3] = HUGE_VAL;
+        s.diag[4] = (double)(s.i + 1);
+        s.ret = 2;
+        s.pc = PC_DONE;
+        break;
+      // ------------------------------------------------------------ ALM
+      case PC_L0:  // alm.c:289-300
+        issue(OP_ALM_LAMPROJ, 0, PC_L1);
+        break;
+      case PC_L1:
+        s.cx = res[0];
+        s.outer = 0;
+        s.precision = 1;
+        s.prev_norm = HUGE_VAL;
+        s.max_viol = HUGE_VAL;
+        s.alm_pg = HUGE_VAL;
+        s.alm_ret = 1;
+        s.alm_diag[0] = s.cx;
+        s.alm_diag[1] = s.max_viol;
+        s.alm_diag[2] = s.alm_pg;
+        s.pc = s.outer_niter ? PC_LB0 : PC_DONE;
+        break;
+      case PC_LB0:  // alm.c:247-257
+        issue(OP_PENALISE, 0, PC_LB1);
+        break;
+      case PC_LB1:
+        s.offset = res[0];
+        issue(OP_PERMUTE_PROBLEM, 0, PC_LB2);
+        break;
+      case PC_LB2:
+        issue(OP_STEP_SIZES, 0, PC_LB3);
+        break;
+      case PC_LB3:
+        s.niter = 4294967295ull;  // -1u (alm.c:255)
+        s.max_pg = fmin(1.0, s.precision);
+        s.max_value = -HUGE_VAL;
+        s.min_delta = s.inner_min_delta;
+        s.period = 10000;
+        s.pc = PC_A_INIT;
+        break;
+      case PC_LE0:  // alm.c:258-264
+        s.inner_reason = s.ret;
+        issue(OP_ALM_VIOL, 0, PC_LE1, 2u);
+        break;
+      case PC_LE1: {  // alm.c:258-285, :194-212
+        s.norm = sqrt(res[0]);
+        s.max_viol = res[1];
+        s.w1_pre = res[2];
+        s.cx = res[3];
+        s.alm_pg = s.diag[2];
+        if (s.log)
+          emit(ACD_EV_ALM_ITER, s.outer + 1, s.max_viol, s.norm, s.diag[2],
+               s.diag[0], s.cx);
+        const bool stalled = (s.inner_reason != 2);
+        s.upd = 0;
+        s.bump = 0;
+        s.scale = 1.0;
+        s.argmax = -1.0;
+        if (!stalled) {
+          if (!(s.norm < .5 * s.prev_norm)) {
+            s.upd = 1;
+            s.scale = 2 * fmin(1.0, s.norm / s.prev_norm);
+            s.bump = s.max_viol > .5 * s.norm;
+          }
+          s.prev_norm = s.norm;
+        }
+        if (s.bump)
+          issue(OP_ARGMAX, 0, PC_LE2, 1u, s.max_viol);
+        else
+          s.pc = PC_LE3;
+        break;
+      }
+      case PC_LE2:
+        s.argmax = -res[0];
+        s.pc = PC_LE3;
+        break;
+      case PC_LE3:
+        s.op_u = (uint32_t)(s.argmax < 0 ? 0xffffffffu : (uint32_t)s.argmax);
+        issue(OP_LAM_W, (s.upd ? F_UPD : 0) | (s.bump ? F_BUMP : 0), PC_LE4, 0,
+              s.scale);
+        break;
+      case PC_LE4: {  // alm.c:235-245, :273-285, :303-319
+        const bool stalled = (s.inner_reason != 2);
+        const double w1 = stalled ? s.w1_pre : res[0];
+        const double wv = (double)a.m / w1;
+        s.precision = fmin(s.precision, fmin(pow(s.max_viol, 1.0), pow(wv, 1.5)));
+        s.precision = fmax(s.precision, s.precision_floor);
+        if (s.log)
+          emit(ACD_EV_ALM_TAIL, s.outer + 1, stalled ? 1.0 : 0.0,
+               res[0] / (double)a.m);
+        s.outer++;
+        s.alm_diag[0] = s.cx;
+        s.alm_diag[1] = s.max_viol;
+        s.alm_diag[2] = s.alm_pg;
+        if (s.max_viol < s.stop_violation && (stalled || s.alm_pg < s.stop_pg)) {
+          s.alm_ret = 0;
+          s.pc = PC_DONE;
+        } else if (s.outer >= s.outer_niter) {
+          s.alm_ret = 1;
+          s.pc = PC_DONE;
+        } else {
+          s.pc = PC_LB0;
+        }
+        break;
+      }
+      // ------------------------------------------------------ approx_update
+      case PC_U0:
+        issue(OP_PERMUTE_PROBLEM, 0, PC_U1);
+        break;
+      case PC_U1:
+        issue(OP_STEP_SIZES, 0, PC_U2);
+        break;
+      case PC_U2:
+        s.pc = PC_DONE;
+        break;
+      default:
+        fail(ACD_ERR_LAUNCH);
+        break;
+    }
+  }
+};
+
+}  // namespace acd

@@ -39,6 +39,16 @@ int acd_spmv_device(const sparse_matrix_t *, int transpose,
 int acd_spmv_device_time(const sparse_matrix_t *, int transpose,
                          double * d_y, const double * d_x, int reps,
                          double * OUT_ms_per_launch);
+/* SpMV kernel variants (unroll depth x slices per wavefront; all give
+ * bit-identical results).  Times variant `variant` like
+ * acd_spmv_device_time; acd_spmv_set_variant makes one the default for
+ * subsequent products. */
+int acd_spmv_variant_time(const sparse_matrix_t *, int transpose,
+                          double * d_y, const double * d_x, int reps,
+                          int variant, double * OUT_ms_per_launch);
+int acd_spmv_set_variant(int variant);
+enum { ACD_SPMV_VARIANTS = 9 };
+
 /* Stored entries of the operator (padding to the slice length included). */
 uint64_t acd_matrix_stored_entries(const sparse_matrix_t *, int transpose);
 
@@ -62,6 +72,11 @@ int alm_solve_ex(alm_t *, size_t niter, double * x, size_t nvars,
                  double * lambda, size_t nconstraints, FILE * log,
                  double * OUT_diagnosis, const struct acd_alm_options *);
 
+/* Restore the row weights (1) and penalised linear term (0) that alm_make
+ * starts from, so the next alm_solve repeats a fresh solve; the original
+ * keeps them across alm_solve calls. */
+int acd_alm_reset(alm_t *);
+
 /* Counters of the most recent alm_solve / approx_solve / mpg_solve. */
 struct acd_solve_stats {
         uint64_t inner_iterations;   /* APPROX / SPG iterations */
@@ -70,6 +85,11 @@ struct acd_solve_stats {
         double device_ms;            /* time inside solver launches */
 };
 void acd_last_solve_stats(struct acd_solve_stats *);
+/* Time split of the last solve by device pass: for pass p < n, the number
+ * of launches and the microseconds from its issue to the next decision.
+ * Pass names are listed by acd_pass_name(). */
+void acd_last_solve_profile(uint64_t * counts, double * us, int n);
+const char * acd_pass_name(int p);
 
 #ifdef __cplusplus
 }

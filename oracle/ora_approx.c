@@ -16,6 +16,16 @@
 #include <string.h>
 
 
+/* Test knob: when set, every reduction below visits its elements in
+ * reverse order (each lane still accumulates sequentially).  The result
+ * is another valid summation order, used by the fixtures to measure how
+ * sensitive a trajectory is to the order of its sums; 0 (the default)
+ * is the reference's order. */
+static int g_reverse;
+void ora_set_reverse_sums(int on) { g_reverse = on; }
+#define FOR_IDX(i, n) for (size_t i##_ = 0, i = g_reverse ? (n) - 1 : 0; \
+                           i##_ < (n); i##_++, i = g_reverse ? (n) - 1 - i##_ : i##_)
+
 static inline double sse_max(double a, double b) { return a > b ? a : b; }
 static inline double sse_min(double a, double b) { return a < b ? a : b; }
 
@@ -192,7 +202,7 @@ static void compute_violation(struct vec * xv, ora_approx_t * a)
         size_t m = a->nrhs;
         ora_sparse_matrix_multiply(xv->viol, m, a->matrix, xv->x, a->nvars, 0);
         double acc[2] = {0, 0};
-        for (size_t i = 0; i < m; i++) {
+        FOR_IDX(i, m) {
                 double v = xv->viol[i] - a->rhs[i];
                 double s = a->weight[i] * v;
                 xv->viol[i] = s;
@@ -206,7 +216,7 @@ static void compute_violation(struct vec * xv, ora_approx_t * a)
 static double dot_pair(const double * x, const double * y, size_t n)
 {
         double acc[2] = {0, 0};
-        for (size_t i = 0; i < n; i++) acc[i & 1] += x[i] * y[i];
+        FOR_IDX(i, n) acc[i & 1] += x[i] * y[i];
         return acc[0] + acc[1];
 }
 
@@ -288,7 +298,7 @@ static double long_step(struct vec * zp, double theta, double len,
         double itheta = (len - 1e-6) / theta;
         double theta2 = theta / len;
         double lin[2] = {0, 0}, quad[2] = {0, 0};
-        for (size_t i = 0; i < zp->n; i++) {
+        FOR_IDX(i, zp->n) {
                 double gi = g->x[i], zi = z->x[i];
                 double s = itheta * a->inv_v[i];
                 double t = zi - gi * s;
@@ -309,7 +319,7 @@ static double dot_diff(const struct vec * g, const struct vec * z,
                        const struct vec * zp)
 {
         double acc[2] = {0, 0};
-        for (size_t i = 0; i < g->n; i++)
+        FOR_IDX(i, g->n)
                 acc[i & 1] += g->x[i] * (zp->x[i] - z->x[i]);
         return acc[0] + acc[1];
 }
@@ -319,7 +329,7 @@ static double pg_norm(const struct vec * g, const struct vec * x,
                       const double * l, const double * u)
 {
         double acc[2] = {0, 0};
-        for (size_t i = 0; i < g->n; i++) {
+        FOR_IDX(i, g->n) {
                 double xi = x->x[i];
                 double xp = xi - g->x[i];
                 xp = sse_max(l[i], xp);
@@ -334,7 +344,7 @@ static double pg_norm(const struct vec * g, const struct vec * x,
 static double diff_norm(const double * x, const double * y, size_t n)
 {
         double acc[2] = {0, 0};
-        for (size_t i = 0; i < n; i++) {
+        FOR_IDX(i, n) {
                 double d = x[i] - y[i];
                 acc[i & 1] += d * d;
         }
@@ -344,7 +354,7 @@ static double diff_norm(const double * x, const double * y, size_t n)
 static double norm2(const struct vec * v)
 {
         double acc[2] = {0, 0};
-        for (size_t i = 0; i < v->n; i++) acc[i & 1] += v->x[i] * v->x[i];
+        FOR_IDX(i, v->n) acc[i & 1] += v->x[i] * v->x[i];
         return sqrt(acc[0] + acc[1]);
 }
 

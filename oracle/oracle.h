@@ -90,6 +90,10 @@ int ora_cd_solve(double * x, size_t n, ora_approx_t *, size_t niter,
                  double max_pg, double max_value, double min_delta,
                  FILE * log, size_t period, double * diag, double offset);
 
+/* Test knob: sum every APPROX/SPG reduction in reverse element order
+ * (another valid summation order); 0 restores the reference's order. */
+void ora_set_reverse_sums(int on);
+
 /* ---- ALM (alm.c) ---- */
 /* Inner solver used by the outer loop. */
 enum { ORA_INNER_APPROX = 0, ORA_INNER_CD = 1 };

@@ -98,6 +98,7 @@ class Oracle:
             "ora_alm_solve": (C.c_int, [vp, sz, dp, sz, dp, sz, vp, dp,
                                         C.POINTER(AlmOptions)]),
             "ora_alm_default_options": (None, [C.POINTER(AlmOptions)]),
+            "ora_set_reverse_sums": (None, [C.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

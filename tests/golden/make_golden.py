@@ -13,7 +13,16 @@ Fixtures (numpy .npz, float64 unless noted):
                     and 1), permutation images of arange, A x, A' y, and
                     the two-RHS products for fixed x/y.
   approx_<case>.npz approx_solve / mpg_solve from x = 0 for several
-                    iteration caps: return code, x, diagnosis, log text.
+                    iteration caps: return code, x, diagnosis, log text,
+                    and how far the reference's own 4-worker thread-pool
+                    run and the oracle summing in reverse order (other
+                    summation orders) land from the serial run: whether it prints the same log, and the relative
+                    deviation of x and of the diagnosis.  Near convergence
+                    APPROX takes restart/step decisions on rounding noise,
+                    and SPG's Barzilai-Borwein step amplifies rounding
+                    (on c0 the pool run's objective after 250 iterations
+                    differs from the serial one entirely), so the summation
+                    order decides those trajectories.
   alm_<case>.npz    alm_solve (APPROX inner solver) from x = 0, lambda = 0:
                     return code, x, lambda, diagnosis, log text.
   cd_<case>.npz     cd_solve after srandom(1) (the as-shipped inner solver).
@@ -67,6 +76,10 @@ def main():
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"],
                    check=True, stdout=subprocess.DEVNULL)
     ref = CLib(pyoracle.REF_APPROX_SO, gpu_extras=False)
+    pool = ref.lib.thread_pool_init(4)
+    ora = pyoracle.Oracle()
+    sys.path.insert(0, os.path.dirname(HERE))
+    from test_solver_gpu import logs_match
     rng = np.random.default_rng(2024)
     for name, lp in spmv_cases().items():
         out = {"digest": np.array(lp.digest())}
@@ -91,6 +104,8 @@ def main():
     for name, lp in solve_cases().items():
         m = ref.make_matrix(lp, 0)
         a = ref.approx_make(m, lp)
+        mo = ora.make_matrix(lp, 0)
+        ao = ora.approx_make(mo, lp)
         out = {"digest": np.array(lp.digest())}
         caps = [1, 2, 9, 10, 99, 100, 101, 250] if name != "c0" else [1, 10, 100, 250]
         out["caps"] = np.array(caps)
@@ -103,6 +118,26 @@ def main():
                 out[f"{solver}_x_{cap}"] = x
                 out[f"{solver}_diag_{cap}"] = diag
                 out[f"{solver}_log_{cap}"] = np.array(log)
+                # Is this trajectory insensitive to the reduction order?
+                # The reference's own 4-worker pool sums in another order.
+                p = fn(a, np.zeros(lp.ncols), cap, log=True, period=10,
+                       offset=0.5, pool=pool)
+                out[f"{solver}_poolstable_{cap}"] = np.array(
+                    bool(logs_match(p[3], log)))
+                # The same with the oracle summing in reverse order (a
+                # pool never splits small vectors).
+                ofn = ora.approx_solve if solver == "approx" else ora.mpg_solve
+                ora.lib.ora_set_reverse_sums(1)
+                rv = ofn(ao, np.zeros(lp.ncols), cap, offset=0.5)
+                ora.lib.ora_set_reverse_sums(0)
+                scale = max(1.0, float(np.max(np.abs(x)))) if x.size else 1.0
+                dscale = np.maximum(1.0, np.abs(diag[:3]))
+                xdev = max(float(np.max(np.abs(p[1] - x))),
+                           float(np.max(np.abs(rv[1] - x)))) / scale
+                ddev = max(float(np.max(np.abs(p[2][:3] - diag[:3]) / dscale)),
+                           float(np.max(np.abs(rv[2][:3] - diag[:3]) / dscale)))
+                out[f"{solver}_pooldev_{cap}"] = np.array(xdev)
+                out[f"{solver}_pooldiagdev_{cap}"] = np.array(ddev)
         # stopping on the projected gradient
         rc, x, diag, log = ref.approx_solve(a, np.zeros(lp.ncols), 100000,
                                             max_pg=1e-3, min_delta=1e-12,
@@ -111,6 +146,8 @@ def main():
                    approx_pg_diag=diag, approx_pg_log=np.array(log))
         save(f"approx_{name}.npz", **out)
         ref.approx_free(a)
+        ora.approx_free(ao)
+        ora.free_matrix(mo)
 
         alm = ref.alm_make(m, lp)
         rc, x, lam, diag, log = ref.alm_solve(alm, 1000, np.zeros(lp.ncols),

@@ -32,23 +32,40 @@ def load(name):
 
 def close(a, b, rtol):
     a, b = np.asarray(a, float), np.asarray(b, float)
-    scale = max(1.0, float(np.max(np.abs(b)))) if b.size else 1.0
-    return a.shape == b.shape and bool(np.all(np.abs(a - b) <= rtol * scale))
+    if a.shape != b.shape:
+        return False
+    same = (a == b) | (np.isnan(a) & np.isnan(b))
+    fin = np.isfinite(b)
+    scale = max(1.0, float(np.max(np.abs(b[fin])))) if fin.any() else 1.0
+    return bool(np.all(same | (np.abs(a - b) <= rtol * scale)))
 
 
-def logs_match(a, b, rtol=1e-5):
-    """Same lines and markers; numbers equal to the printed precision."""
+def logs_match(a, b, rtol=1e-5, atol=1e-9):
+    """Same lines and markers (whitespace-insensitive: %12g pads by the
+    number of printed digits); numbers equal to the printed precision,
+    values below atol being rounding noise."""
     la, lb = a.splitlines(), b.splitlines()
     if len(la) != len(lb):
         return False
     for x, y in zip(la, lb):
-        if NUM.sub("#", x) != NUM.sub("#", y):
+        tx, ty = x.split(), y.split()
+        if len(tx) != len(ty):
             return False
-        for p, q in zip(NUM.findall(x), NUM.findall(y)):
-            fp, fq = float(p), float(q)
-            if not (fp == fq or abs(fp - fq) <= rtol * max(1.0, abs(fq))):
+        for p, q in zip(tx, ty):
+            if NUM.fullmatch(p) and NUM.fullmatch(q):
+                fp, fq = float(p), float(q)
+                if not (fp == fq or abs(fp - fq) <= rtol * max(1.0, abs(fq))
+                        or (abs(fp) < atol and abs(fq) < atol)):
+                    return False
+            elif p != q:
                 return False
     return True
+
+
+def log_diff(got, want):
+    import difflib
+    return "\n".join(difflib.unified_diff(want.splitlines(), got.splitlines(),
+                                          "reference", "gpu", lineterm=""))
 
 
 @pytest.fixture(scope="module")
@@ -73,11 +90,22 @@ def test_inner_solver_matches_reference(gpu, problems, name, solver):
         print(f"[solver] {solver} {name} niter={cap}", flush=True)
         rc, x, diag, log = fn(a, np.zeros(lp.ncols), int(cap), log=True,
                               period=10, offset=0.5)
+        # Tolerance: 1e-9, or a thousand times how far the reference's own
+        # thread-pool run moves from its serial run (make_golden.py).
+        xtol = max(1e-9, 1e3 * float(f[f"{solver}_pooldev_{cap}"]))
+        dtol = max(1e-9, 1e3 * float(f[f"{solver}_pooldiagdev_{cap}"]))
         assert rc == int(f[f"{solver}_rc_{cap}"])
-        assert close(x, f[f"{solver}_x_{cap}"], 1e-9), cap
-        assert close(diag[:4], f[f"{solver}_diag_{cap}"][:4], 1e-9), (diag, f[f"{solver}_diag_{cap}"])
+        assert close(x, f[f"{solver}_x_{cap}"], xtol), cap
+        assert close(diag[:4], f[f"{solver}_diag_{cap}"][:4], dtol), (diag, f[f"{solver}_diag_{cap}"])
         assert diag[4] == f[f"{solver}_diag_{cap}"][4]
-        assert logs_match(log, str(f[f"{solver}_log_{cap}"])), (log, str(f[f"{solver}_log_{cap}"]))
+        # Logs must agree wherever the trajectory does not depend on the
+        # summation order: the reference's own thread-pool run prints the
+        # same log and neither it nor the reversed-order oracle moves x
+        # (see tests/golden/make_golden.py).
+        if (bool(f[f"{solver}_poolstable_{cap}"])
+                and float(f[f"{solver}_pooldev_{cap}"]) <= 1e-12):
+            want = str(f[f"{solver}_log_{cap}"])
+            assert logs_match(log, want), log_diff(log, want)
 
 
 @pytest.mark.parametrize("name", ["small", "setcover", "mcf"])

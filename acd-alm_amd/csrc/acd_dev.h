@@ -50,6 +50,7 @@ struct acd_sell {
         uint32_t * col;        /* device [nent] */
         double * val;          /* device [nent] */
         float * val32;         /* device [nent] or NULL (mixed precision) */
+        uint16_t * val16;      /* device [nent] bf16 bits, or NULL */
         uint32_t * rowlen;     /* device [nrows]: entries per row */
 };
 
@@ -74,8 +75,12 @@ int acd_dev_matrix_build(struct acd_dev_matrix * out,
                          uint32_t * h_row_idx, uint32_t * h_row_ridx,
                          uint32_t * h_col_idx, uint32_t * h_col_ridx);
 void acd_dev_matrix_free(struct acd_dev_matrix *);
-/* FP32 copy of the stored values (exact for 0/+-1 matrices). */
+/* FP32 copy of the stored values (rounded to nearest). */
 int acd_dev_sell_make_fp32(struct acd_sell *);
+/* BF16 copy of the stored values, made only when every value is exactly
+ * representable (0/+-1 and small dyadic coefficients).  Returns 1 when the
+ * copy exists, 0 when some value is not exact (no copy), -1 on error. */
+int acd_dev_sell_make_bf16(struct acd_sell *);
 
 /* ---- vector kernels (dev_spmv.hip) ---- */
 /* y = op x on device pointers (op = A' when transpose). */
@@ -114,6 +119,8 @@ enum acd_vec {
 };
 
 enum acd_mode { ACD_MODE_APPROX = 1, ACD_MODE_MPG = 2, ACD_MODE_ALM = 3 };
+/* Working-copy precision (same values as acd_gpu.h's enum acd_precision). */
+enum { ACD_DEV_FP64 = 0, ACD_DEV_MIXED = 1, ACD_DEV_FP32 = 2 };
 
 struct acd_solve_request {
         int mode;                 /* enum acd_mode */
@@ -123,7 +130,7 @@ struct acd_solve_request {
         uint64_t period;
         /* ALM only */
         double stop_violation, stop_pg, precision_floor, inner_min_delta;
-        int precision;            /* enum acd_precision */
+        int precision;            /* ACD_DEV_FP64 / _MIXED / _FP32 */
 };
 
 struct acd_solve_result {

@@ -172,6 +172,19 @@ __global__ void k_to_f32(const double* src, float* dst, uint64_t n) {
   if (i < n) dst[i] = (float)src[i];
 }
 
+// bf16 = the high half of the fp32 bit pattern; flags any value whose
+// double -> fp32 -> bf16 truncation is not exact.
+__global__ void k_to_bf16(const double* src, uint16_t* dst, uint32_t* inexact,
+                          uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double d = src[i];
+  const float f = (float)d;
+  const uint32_t bits = __float_as_uint(f);
+  dst[i] = (uint16_t)(bits >> 16);
+  if ((double)f != d || (bits & 0xffffu)) atomicOr(inexact, 1u);
+}
+
 struct Scratch {
   void* p = nullptr;
   size_t bytes = 0;
@@ -327,6 +340,7 @@ void free_sell(struct acd_sell* op) {
   hipFree(op->col);
   hipFree(op->val);
   hipFree(op->val32);
+  hipFree(op->val16);
   hipFree(op->rowlen);
   memset(op, 0, sizeof *op);
 }
@@ -484,6 +498,38 @@ int acd_dev_sell_make_fp32(struct acd_sell* op) {
                                                          op->nent);
   CK(hipStreamSynchronize(S()), "fp32 conversion");
   return 0;
+}
+
+int acd_dev_sell_make_bf16(struct acd_sell* op) {
+  if (op->val16) return 1;
+  uint16_t* v16 = nullptr;
+  uint32_t* inexact = nullptr;
+  CK(hipMalloc((void**)&v16, sizeof(uint16_t) * (op->nent ? op->nent : 1)),
+     "hipMalloc bf16 values");
+  if (hipMalloc((void**)&inexact, sizeof(uint32_t)) != hipSuccess) {
+    hipFree(v16);
+    acd_dev_record_error(-1, "hipMalloc bf16 flag");
+    return -1;
+  }
+  uint32_t h = 0;
+  hipMemsetAsync(inexact, 0, sizeof(uint32_t), S());
+  if (op->nent)
+    k_to_bf16<<<blocks_for(op->nent, 256), 256, 0, S()>>>(op->val, v16,
+                                                          inexact, op->nent);
+  hipMemcpyAsync(&h, inexact, sizeof h, hipMemcpyDeviceToHost, S());
+  if (hipStreamSynchronize(S()) != hipSuccess) {
+    hipFree(v16);
+    hipFree(inexact);
+    acd_dev_record_error(-1, "bf16 conversion");
+    return -1;
+  }
+  hipFree(inexact);
+  if (h) {
+    hipFree(v16);
+    return 0;
+  }
+  op->val16 = v16;
+  return 1;
 }
 
 }  // extern "C"

@@ -24,6 +24,15 @@ inline hipStream_t S() { return (hipStream_t)acd_dev_stream(); }
 constexpr int kWPB = 4;  // wavefronts per workgroup of the step kernel
 constexpr uint32_t kEventCap = 4096;
 constexpr int kPairs = 32;  // (step, control) pairs per graph
+// Precision keys: iterate vectors / constant vectors / matrix values.
+enum {
+  K_F64 = 0,    // double / double / double
+  K_MIX_F32,    // double / float / float
+  K_MIX_BF16,   // double / float / bf16
+  K_F32,        // float / float / float
+  K_F32_BF16,   // float / float / bf16
+  kKeys
+};
 uint64_t g_prof_count[32];
 double g_prof_us[32];
 
@@ -38,6 +47,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
   return (b % 8) * (nb / 8) + b / 8;
 }
 
+template <class VI, class VC, class MT>
 __global__ void __launch_bounds__(64 * kWPB) k_step(Args a) {
   const State& s = *a.st;
   if (s.op == OP_NONE) return;
@@ -45,7 +55,7 @@ __global__ void __launch_bounds__(64 * kWPB) k_step(Args a) {
   g.lane = threadIdx.x & 63;
   g.gwave = xcd_block(blockIdx.x, gridDim.x) * kWPB + (threadIdx.x >> 6);
   g.nwaves = gridDim.x * kWPB;
-  Ops<GridG> ops(g, a, s);
+  Ops<GridG, VI, VC, MT> ops(g, a, s);
   ops.run();
   const uint32_t mm = s.op_maxmask;
   __shared__ double lds[kWPB][kRes];
@@ -111,6 +121,8 @@ Sell view(const acd_sell& op) {
   v.off = op.off;
   v.col = op.col;
   v.val = op.val;
+  v.val32 = op.val32;
+  v.val16 = op.val16;
   v.nrows = op.nrows;
   v.nslices = op.nslices;
   return v;
@@ -131,8 +143,9 @@ struct acd_dev_lp {
   double* partials;
   acd_event* ev;
   int blocks;
-  hipGraphExec_t graph;
-  int have_graph;
+  hipGraphExec_t graph[kKeys];
+  int have_graph[kKeys];
+  int pkey;  // precision key the permuted problem was last written in
 };
 
 namespace {
@@ -191,8 +204,23 @@ int pick_blocks(const acd_dev_matrix* orig, const acd_dev_matrix* perm) {
   return want;
 }
 
-int build_graph(acd_dev_lp* lp) {
-  if (lp->have_graph) return 0;
+void launch_step(int key, int blocks, hipStream_t st, const Args& a) {
+  const dim3 grid(blocks), block(64 * kWPB);
+  switch (key) {
+    case K_MIX_F32: k_step<double, float, float><<<grid, block, 0, st>>>(a); break;
+    case K_MIX_BF16:
+      k_step<double, float, uint16_t><<<grid, block, 0, st>>>(a);
+      break;
+    case K_F32: k_step<float, float, float><<<grid, block, 0, st>>>(a); break;
+    case K_F32_BF16:
+      k_step<float, float, uint16_t><<<grid, block, 0, st>>>(a);
+      break;
+    default: k_step<double, double, double><<<grid, block, 0, st>>>(a); break;
+  }
+}
+
+int build_graph(acd_dev_lp* lp, int key) {
+  if (lp->have_graph[key]) return 0;
   hipStream_t cs;
   if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
     return -1;
@@ -204,12 +232,12 @@ int build_graph(acd_dev_lp* lp) {
     rc = -1;
   } else {
     for (int i = 0; i < kPairs; i++) {
-      k_step<<<lp->blocks, 64 * kWPB, 0, cs>>>(a);
+      launch_step(key, lp->blocks, cs, a);
       k_ctl<<<1, 256, 0, cs>>>(a, (uint32_t)lp->blocks);
     }
     if (hipStreamEndCapture(cs, &g) != hipSuccess) rc = -1;
-    if (!rc && hipGraphInstantiate(&lp->graph, g, nullptr, nullptr, 0) !=
-                   hipSuccess)
+    if (!rc && hipGraphInstantiate(&lp->graph[key], g, nullptr, nullptr,
+                                   0) != hipSuccess)
       rc = -1;
     if (!rc) hipGraphDestroy(g);
   }
@@ -218,7 +246,7 @@ int build_graph(acd_dev_lp* lp) {
     acd_dev_record_error(rc, "solver graph capture");
     return rc;
   }
-  lp->have_graph = 1;
+  lp->have_graph[key] = 1;
   return 0;
 }
 
@@ -240,6 +268,7 @@ void init_state(State* s, const acd_dev_lp* lp, const acd_solve_request* q) {
   s->stop_pg = q->stop_pg;
   s->precision_floor = q->precision_floor;
   s->inner_min_delta = q->inner_min_delta;
+  s->mixed = q->precision == ACD_DEV_FP32;
   s->op = OP_NONE;
   switch (q->mode) {
     case ACD_MODE_APPROX: s->pc = PC_A_INIT; break;
@@ -252,9 +281,9 @@ void init_state(State* s, const acd_dev_lp* lp, const acd_solve_request* q) {
   }
 }
 
-int run_to_completion(acd_dev_lp* lp, State* hs, acd_solve_result* res,
-                      acd_event_sink sink, void* ctx) {
-  if (build_graph(lp)) return -1;
+int run_to_completion(acd_dev_lp* lp, int key, State* hs,
+                      acd_solve_result* res, acd_event_sink sink, void* ctx) {
+  if (build_graph(lp, key)) return -1;
   Args a = make_args(lp);
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
@@ -270,7 +299,7 @@ int run_to_completion(acd_dev_lp* lp, State* hs, acd_solve_result* res,
   int chunk = 1;
   while (!rc) {
     for (int c = 0; c < chunk && !rc; c++) {
-      if (hipGraphLaunch(lp->graph, S()) != hipSuccess) rc = -1;
+      if (hipGraphLaunch(lp->graph[key], S()) != hipSuccess) rc = -1;
       launches += 2 * kPairs;
     }
     if (rc) {
@@ -333,6 +362,22 @@ int run_to_completion(acd_dev_lp* lp, State* hs, acd_solve_result* res,
   return rc;
 }
 
+// ACD_DEV_MIXED keeps the iterates in FP64 and the working copy's constant
+// vectors in FP32; ACD_DEV_FP32 also stores the iterates in FP32.  Both
+// read matrix values in BF16 when every value is exact there, else FP32.
+int precision_key(acd_dev_lp* lp, int precision) {
+  if (precision != ACD_DEV_MIXED && precision != ACD_DEV_FP32) return K_F64;
+  acd_dev_matrix* pm = const_cast<acd_dev_matrix*>(lp->perm);
+  const int ba = acd_dev_sell_make_bf16(&pm->a);
+  const int bt = acd_dev_sell_make_bf16(&pm->at);
+  if (ba < 0 || bt < 0) return -1;
+  const bool bf = ba == 1 && bt == 1;
+  if (!bf && (acd_dev_sell_make_fp32(&pm->a) || acd_dev_sell_make_fp32(&pm->at)))
+    return -1;
+  if (precision == ACD_DEV_FP32) return bf ? K_F32_BF16 : K_F32;
+  return bf ? K_MIX_BF16 : K_MIX_F32;
+}
+
 }  // namespace
 
 extern "C" {
@@ -385,7 +430,8 @@ struct acd_dev_lp* acd_dev_lp_create(const struct acd_dev_matrix* orig,
 
 void acd_dev_lp_free(struct acd_dev_lp* lp) {
   if (!lp) return;
-  if (lp->have_graph) hipGraphExecDestroy(lp->graph);
+  for (int k = 0; k < kKeys; k++)
+    if (lp->have_graph[k]) hipGraphExecDestroy(lp->graph[k]);
   for (int v = 0; v < ACD_V_COUNT; v++) acd_dev_free(lp->vec[v]);
   acd_dev_free(lp->r);
   acd_dev_free(lp->p_rhs);
@@ -413,15 +459,32 @@ int acd_dev_approx_update(struct acd_dev_lp* lp) {
   State hs;
   init_state(&hs, lp, &q);
   acd_solve_result res;
-  return run_to_completion(lp, &hs, &res, nullptr, nullptr);
+  const int rc = run_to_completion(lp, K_F64, &hs, &res, nullptr, nullptr);
+  lp->pkey = K_F64;
+  return rc;
 }
 
 int acd_dev_solve(struct acd_dev_lp* lp, const struct acd_solve_request* q,
                   struct acd_solve_result* res, acd_event_sink sink,
                   void* ctx) {
+  const int key = precision_key(lp, q->precision);
+  if (key < 0) return -1;
+  // APPROX/SPG solve the permuted problem as last written; rewrite it if
+  // it was written in another precision.  ALM rewrites it every outer
+  // iteration.
+  if (q->mode != ACD_MODE_ALM && key != lp->pkey) {
+    acd_solve_request u;
+    memset(&u, 0, sizeof u);
+    State hu;
+    init_state(&hu, lp, &u);
+    if (run_to_completion(lp, key, &hu, nullptr, nullptr, nullptr)) return -1;
+    lp->pkey = key;
+  }
   State hs;
   init_state(&hs, lp, q);
-  return run_to_completion(lp, &hs, res, sink, ctx);
+  const int rc = run_to_completion(lp, key, &hs, res, sink, ctx);
+  if (q->mode == ACD_MODE_ALM) lp->pkey = key;
+  return rc;
 }
 
 }  // extern "C"

@@ -35,6 +35,9 @@
 namespace acd {
 
 constexpr int kRes = 8;  // result slots per op
+// Relative checkpoint change (over 100 iterations) below which FP32
+// iterates count as stalled: 2^-20, about 8 ulps of FP32.
+constexpr double kMixedMinDelta = 9.5367431640625e-07;
 
 // n-side and m-side work vector roles.
 enum { NX = 0, NZ, NZP, NG, NG2, NPV, kNumVN };
@@ -103,9 +106,28 @@ enum Pc : int32_t {
 struct Sell {
   const uint64_t* off;
   const uint32_t* col;
-  const double* val;
+  const double* val;       // FP64 values
+  const float* val32;      // FP32 copy (mixed precision) or NULL
+  const uint16_t* val16;   // BF16 copy when every value is exact, or NULL
   uint32_t nrows, nslices;
 };
+
+// Value of entry e as stored for precision MT (double, float, or bf16
+// carried in a uint16_t), widened to double for the FP64 accumulation.
+template <class MT>
+__device__ __forceinline__ double mval(const Sell& M, uint64_t e);
+template <>
+__device__ __forceinline__ double mval<double>(const Sell& M, uint64_t e) {
+  return M.val[e];
+}
+template <>
+__device__ __forceinline__ double mval<float>(const Sell& M, uint64_t e) {
+  return (double)M.val32[e];
+}
+template <>
+__device__ __forceinline__ double mval<uint16_t>(const Sell& M, uint64_t e) {
+  return (double)__uint_as_float((uint32_t)M.val16[e] << 16);
+}
 
 struct State {
   // control
@@ -132,6 +154,7 @@ struct State {
   uint64_t outer, outer_niter;
   double precision, prev_norm, max_viol, alm_pg, cx, norm, w1_pre;
   double stop_violation, stop_pg, precision_floor, inner_min_delta;
+  int32_t mixed;  // FP32 iterates: inner stall test at the FP32 floor
   double alm_diag[3];
   int32_t alm_ret, inner_reason, upd, bump;
   double scale, argmax;
@@ -177,8 +200,12 @@ __device__ __forceinline__ double wave_reduce(double v, bool is_max) {
 
 // ---------------------------------------------------------------------------
 // Ops.  G supplies lane, gwave (this wave's index among the group's waves)
-// and nwaves; slice s of 64 rows goes to wave s mod nwaves.
-template <class G>
+// and nwaves; slice s of 64 rows goes to wave s mod nwaves.  V is the type
+// of the working-copy vectors (double, or float in the mixed form) and MT
+// that of the working-copy matrix values (double, float or bf16); every
+// product is formed and accumulated in double.  The ALM-level passes over
+// the caller's matrix and vectors stay in double.
+template <class G, class VI = double, class VC = double, class MT = double>
 struct Ops {
   const G& g;
   const Args& a;
@@ -191,24 +218,27 @@ struct Ops {
     for (int k = 0; k < kRes; k++) p[k] = 0.0;
   }
 
-  __device__ __forceinline__ double* VN(int r) const { return s.pn[r]; }
-  __device__ __forceinline__ double* VM(int r) const { return s.pm[r]; }
+  __device__ __forceinline__ VI* VN(int r) const { return (VI*)s.pn[r]; }
+  __device__ __forceinline__ VI* VM(int r) const { return (VI*)s.pm[r]; }
+  __device__ __forceinline__ VC* P(double* q) const { return (VC*)q; }
 
   // Row r of M summed lane-sequentially in ascending column order from
   // +0.0 with separate multiply and add (the reference's per-row order,
-  // spmv/spmv_block.c:160-254); epi(row, acc0, acc1).
-  template <int NRHS, class E>
-  __device__ __forceinline__ void spmv_rows(const Sell& M, const double* x0,
-                                            const double* x1, E&& epi) {
+  // spmv/spmv_block.c:160-254); epi(row, acc0, acc1).  MT2/X are the value
+  // and gathered-vector types (the ALM passes use double for both).
+  template <int NRHS, class MT2, class X, class E>
+  __device__ __forceinline__ void spmv_rows_t(const Sell& M, const X* x0,
+                                              const X* x1, E&& epi) {
     for (uint32_t sl = g.gwave; sl < M.nslices; sl += g.nwaves) {
       const uint64_t base = M.off[sl];
       const uint32_t len = (uint32_t)((M.off[sl + 1] - base) >> 6);
-      const double* v = M.val + base + g.lane;
-      const uint32_t* c = M.col + base + g.lane;
+      uint64_t e = base + g.lane;
+      const uint32_t* c = M.col + e;
       double a0 = 0.0, a1 = 0.0;
       uint32_t k = 0;
-      for (; k + 4 <= len; k += 4, v += 256, c += 256) {
-        const double v0 = v[0], v1 = v[64], v2 = v[128], v3 = v[192];
+      for (; k + 4 <= len; k += 4, e += 256, c += 256) {
+        const double v0 = mval<MT2>(M, e), v1 = mval<MT2>(M, e + 64),
+                     v2 = mval<MT2>(M, e + 128), v3 = mval<MT2>(M, e + 192);
         const uint32_t c0 = c[0], c1 = c[64], c2 = c[128], c3 = c[192];
         const double p0 = x0[c0], p1 = x0[c1], p2 = x0[c2], p3 = x0[c3];
         a0 = a0 + v0 * p0;
@@ -223,15 +253,27 @@ struct Ops {
           a1 = a1 + v3 * q3;
         }
       }
-      for (; k < len; k++, v += 64, c += 64) {
-        const double vv = *v;
+      for (; k < len; k++, e += 64, c += 64) {
+        const double vv = mval<MT2>(M, e);
         const uint32_t cc = *c;
-        a0 = a0 + vv * x0[cc];
-        if (NRHS == 2) a1 = a1 + vv * x1[cc];
+        a0 = a0 + vv * (double)x0[cc];
+        if (NRHS == 2) a1 = a1 + vv * (double)x1[cc];
       }
       const uint32_t r = sl * 64 + g.lane;
       if (r < M.nrows) epi(r, a0, a1);
     }
+  }
+  // Working-copy operators (A, A') in the solver precision.
+  template <int NRHS, class E>
+  __device__ __forceinline__ void spmv_rows(const Sell& M, const VI* x0,
+                                            const VI* x1, E&& epi) {
+    spmv_rows_t<NRHS, MT, VI>(M, x0, x1, epi);
+  }
+  // Caller's operators (A0, A0') in double.
+  template <int NRHS, class E>
+  __device__ __forceinline__ void spmv_rows64(const Sell& M, const double* x0,
+                                              const double* x1, E&& epi) {
+    spmv_rows_t<NRHS, double, double>(M, x0, x1, epi);
   }
 
   template <class F>
@@ -243,88 +285,98 @@ struct Ops {
     }
   }
 
+  // Clamp to [l, u] as the reference does: maxpd(l, t) then minpd(u, t).
+  __device__ __forceinline__ double clamp_lu(uint32_t j, double t) const {
+    t = maxpd((double)P(a.p_l)[j], t);
+    return minpd((double)P(a.p_u)[j], t);
+  }
+
   __device__ __forceinline__ void run() {
     const uint32_t fl = s.op_flags;
+    const VC* pc = P(a.p_c);
     switch (s.op) {
       case OP_INIT_X: {  // approx/approx.c:591-599, :534-537
-        double* X = VN(NX);
-        double* Z = VN(NZ);
-        double* PV = VN(NPV);
+        VI* X = VN(NX);
+        VI* Z = VN(NZ);
+        VI* PV = VN(NPV);
         each(a.n, [&](uint32_t j) {
           double xj = a.x_io[a.col_idx[j]];
-          const double lo = maxpd(a.p_l[j], xj);
-          xj = minpd(lo, a.p_u[j]);
-          X[j] = xj;
-          Z[j] = xj;
+          const double lo = maxpd((double)P(a.p_l)[j], xj);
+          xj = minpd(lo, (double)P(a.p_u)[j]);
+          const VI xv = (VI)xj;
+          X[j] = xv;
+          Z[j] = xv;
           if (fl & F_APPROX) {
-            PV[j] = xj;
-            p[0] += a.p_c[j] * xj;
+            PV[j] = xv;
+            p[0] += (double)pc[j] * (double)xv;
           }
         });
         break;
       }
       case OP_VIOL: {  // approx/gradient_value.inc:15-81
-        const double* X = VN(s.op_r0);
-        double* V = VM(s.op_r1);
-        double* c1 = VM(MZV);
-        double* c2 = VM(MYV);
+        const VI* X = VN(s.op_r0);
+        VI* Vv = VM(s.op_r1);
+        VI* c1 = VM(MZV);
+        VI* c2 = VM(MYV);
         const bool copy2 = fl & F_COPY2;
+        const VC* b = P(a.p_rhs);
+        const VC* w = P(a.p_w);
         spmv_rows<1>(a.A, X, nullptr, [&](uint32_t i, double ax, double) {
-          const double v = ax - a.p_rhs[i];
-          const double sv = a.p_w[i] * v;
-          V[i] = sv;
+          const double v = ax - (double)b[i];
+          const double sv = (double)w[i] * v;
+          const VI svv = (VI)sv;
+          Vv[i] = svv;
           if (copy2) {
-            c1[i] = sv;
-            c2[i] = sv;
+            c1[i] = svv;
+            c2[i] = svv;
           }
           p[0] += sv * v;
         });
-        if (fl & F_CX) each(a.n, [&](uint32_t j) { p[1] += a.p_c[j] * X[j]; });
+        if (fl & F_CX)
+          each(a.n, [&](uint32_t j) { p[1] += (double)pc[j] * (double)X[j]; });
         break;
       }
       case OP_PASS1: {  // step_project.inc:179-236, :320-375, :452-508
-        double* Zp = VN(NZP);
-        const double* Z = VN(NZ);
-        double* G1 = VN(NG);
-        double* G2 = VN(NG2);
-        double* X = VN(NX);
-        double* PV = VN(NPV);
+        VI* Zp = VN(NZP);
+        const VI* Z = VN(NZ);
+        VI* G1 = VN(NG);
+        VI* G2 = VN(NG2);
+        VI* X = VN(NX);
+        VI* PV = VN(NPV);
+        const VC* iv = P(a.p_iv);
+        const VC* vv = P(a.p_v);
         const bool two = fl & F_TWO, lng = fl & F_LONG, want_pg = fl & F_PG;
         const bool lazy = fl & F_LAZY, pcopy = fl & F_PCOPY;
         const double itheta = s.op_a, theta2 = s.op_b;
         const double lt = s.op_c, ls = 1 - s.op_c;
         auto epi = [&](uint32_t j, double gy, double gz) {
-          const double cj = a.p_c[j];
-          const double zj = Z[j];
+          const double cj = (double)pc[j];
+          const double zj = (double)Z[j];
           if (pcopy || lazy) {
-            const double xj = X[j];
+            const VI xj = X[j];
             if (pcopy) PV[j] = xj;
-            if (lazy) X[j] = ls * xj + lt * zj;
+            if (lazy) X[j] = (VI)(ls * (double)xj + lt * zj);
           }
           const double g2 = gy + cj;
-          G2[j] = g2;
-          const double stp = itheta * a.p_iv[j];
-          double t = zj - g2 * stp;
-          t = maxpd(a.p_l[j], t);
-          t = minpd(a.p_u[j], t);
-          Zp[j] = t;
+          G2[j] = (VI)g2;
+          const double stp = itheta * (double)iv[j];
+          const VI tv = (VI)clamp_lu(j, zj - g2 * stp);
+          const double t = (double)tv;
+          Zp[j] = tv;
           p[4] = maxpd(p[4], fabs(t));
           p[2] += cj * zj;
           p[3] += cj * t;
           if (lng) {
             const double d = t - zj;
             p[0] += g2 * d;
-            p[1] += d * d * theta2 * a.p_v[j];
+            p[1] += d * d * theta2 * (double)vv[j];
           }
           if (two) {
             const double gg = gz + cj;
-            G1[j] = gg;
+            G1[j] = (VI)gg;
             p[5] += gg * (t - zj);
             if (want_pg) {
-              double xp = zj - gg;
-              xp = maxpd(a.p_l[j], xp);
-              xp = minpd(a.p_u[j], xp);
-              const double d = zj - xp;
+              const double d = zj - clamp_lu(j, zj - gg);
               p[6] += d * d;
             }
           }
@@ -336,22 +388,25 @@ struct Ops {
         break;
       }
       case OP_PASS2: {  // compute_violation(zp) + approx/vector.inc:249-295
-        const double* Zp = VN(NZP);
-        double* ZPV = VM(MZPV);
-        double* XV = VM(MXV);
-        double* YV = VM(MYV);
-        double* SX = VM(MSX);
-        double* SY = VM(MSY);
+        const VI* Zp = VN(NZP);
+        VI* ZPV = VM(MZPV);
+        VI* XV = VM(MXV);
+        VI* YV = VM(MYV);
+        VI* SX = VM(MSX);
+        VI* SY = VM(MSY);
+        const VC* b = P(a.p_rhs);
+        const VC* w = P(a.p_w);
         const bool spec = fl & F_SPEC;
         const double th = s.op_a, thn = s.op_b;
         const double sc = 1 - th, nsc = 1 - thn;
         spmv_rows<1>(a.A, Zp, nullptr, [&](uint32_t i, double ax, double) {
-          const double v = ax - a.p_rhs[i];
-          const double sv = a.p_w[i] * v;
-          ZPV[i] = sv;
+          const double v = ax - (double)b[i];
+          const VI svv = (VI)((double)w[i] * v);
+          const double sv = (double)svv;
+          ZPV[i] = svv;
           p[0] += sv * v;
-          const double xv = sc * XV[i] + th * sv;
-          const double yv = nsc * xv + thn * sv;
+          const VI xv = (VI)(sc * (double)XV[i] + th * sv);
+          const VI yv = (VI)(nsc * (double)xv + thn * sv);
           if (spec) {
             SX[i] = xv;
             SY[i] = yv;
@@ -363,31 +418,31 @@ struct Ops {
         break;
       }
       case OP_FIXUP: {  // approx/approx.c:383-389, :445-450
-        double* Zp = VN(NZP);
-        const double* Z = VN(NZ);
-        double* G1 = VN(NG);
-        const double* G2 = VN(NG2);
+        VI* Zp = VN(NZP);
+        const VI* Z = VN(NZ);
+        VI* G1 = VN(NG);
+        const VI* G2 = VN(NG2);
+        const VC* iv = P(a.p_iv);
         const bool needg = fl & F_NEEDG, restep = fl & F_RESTEP;
         const double itheta = s.op_a;
         auto body = [&](uint32_t j, double gz) {
-          const double zj = Z[j];
+          const double zj = (double)Z[j];
           double gg;
           if (needg) {
-            gg = gz + a.p_c[j];
-            G1[j] = gg;
+            gg = gz + (double)pc[j];
+            G1[j] = (VI)gg;
           } else {
-            gg = G1[j];
+            gg = (double)G1[j];
           }
           double t;
           if (restep) {
-            const double stp = itheta * a.p_iv[j];
-            t = zj - G2[j] * stp;
-            t = maxpd(a.p_l[j], t);
-            t = minpd(a.p_u[j], t);
-            Zp[j] = t;
+            const double stp = itheta * (double)iv[j];
+            const VI tv = (VI)clamp_lu(j, zj - (double)G2[j] * stp);
+            Zp[j] = tv;
+            t = (double)tv;
             p[1] = maxpd(p[1], fabs(t));
           } else {
-            t = Zp[j];
+            t = (double)Zp[j];
           }
           p[0] += gg * (t - zj);
         };
@@ -399,121 +454,121 @@ struct Ops {
         break;
       }
       case OP_RESTART_COPY: {  // approx/approx.c:452-453
-        double* X = VN(NX);
-        const double* Z = VN(NZ);
-        double* XV = VM(MXV);
-        double* YV = VM(MYV);
-        const double* ZV = VM(MZV);
+        VI* X = VN(NX);
+        const VI* Z = VN(NZ);
+        VI* XV = VM(MXV);
+        VI* YV = VM(MYV);
+        const VI* ZV = VM(MZV);
         each(a.n, [&](uint32_t j) { X[j] = Z[j]; });
         each(a.m, [&](uint32_t r) {
-          const double v = ZV[r];
+          const VI v = ZV[r];
           XV[r] = v;
           YV[r] = v;
         });
         break;
       }
       case OP_CKPT_X: {  // approx/approx.c:641-651
-        double* X = VN(NX);
-        const double* Z = VN(NZ);
-        const double* PV = VN(NPV);
+        VI* X = VN(NX);
+        const VI* Z = VN(NZ);
+        const VI* PV = VN(NPV);
         const bool lazy = fl & F_LAZY;
         const double lt = s.op_c, ls = 1 - s.op_c;
         each(a.n, [&](uint32_t j) {
-          double xj = X[j];
+          double xj = (double)X[j];
           if (lazy) {
-            xj = ls * xj + lt * Z[j];
-            X[j] = xj;
+            const VI xv = (VI)(ls * xj + lt * (double)Z[j]);
+            X[j] = xv;
+            xj = (double)xv;
           }
-          p[0] += a.p_c[j] * xj;
-          const double d = PV[j] - xj;
+          p[0] += (double)pc[j] * xj;
+          const double d = (double)PV[j] - xj;
           p[1] += d * d;
           p[2] += xj * xj;
         });
         break;
       }
       case OP_GRAD_PG: {  // gradient_value.inc:158-181, approx.c:258-278
-        const double* V = VM(s.op_r1);
-        const double* X = VN(s.op_r0);
-        double* Gout = VN(NG);
-        spmv_rows<1>(a.AT, V, nullptr, [&](uint32_t j, double gx, double) {
-          const double gg = gx + a.p_c[j];
-          Gout[j] = gg;
-          const double xj = X[j];
-          double xp = xj - gg;
-          xp = maxpd(a.p_l[j], xp);
-          xp = minpd(a.p_u[j], xp);
-          const double d = xj - xp;
+        const VI* Vv = VM(s.op_r1);
+        const VI* X = VN(s.op_r0);
+        VI* Gout = VN(NG);
+        spmv_rows<1>(a.AT, Vv, nullptr, [&](uint32_t j, double gx, double) {
+          const double gg = gx + (double)pc[j];
+          Gout[j] = (VI)gg;
+          const double xj = (double)X[j];
+          const double d = xj - clamp_lu(j, xj - gg);
           p[0] += d * d;
           p[1] += gg * gg;
         });
         break;
       }
       case OP_NORM_G: {
-        const double* Gv = VN(NG);
-        each(a.n, [&](uint32_t j) { p[0] += Gv[j] * Gv[j]; });
+        const VI* Gv = VN(NG);
+        each(a.n, [&](uint32_t j) {
+          const double gj = (double)Gv[j];
+          p[0] += gj * gj;
+        });
         break;
       }
       case OP_FIN_A: {  // approx/approx.c:681-683
-        const double* C = VN(s.op_r0);
-        const double* X = VN(NX);
-        const double* PV = VN(NPV);
+        const VI* C = VN(s.op_r0);
+        const VI* X = VN(NX);
+        const VI* PV = VN(NPV);
         const bool pcopy = fl & F_PCOPY;
         each(a.n, [&](uint32_t j) {
-          const double cj = C[j];
-          const double pv = pcopy ? X[j] : PV[j];
+          const double cj = (double)C[j];
+          const double pv = pcopy ? (double)X[j] : (double)PV[j];
           const double d = pv - cj;
           p[0] += d * d;
           p[1] += cj * cj;
-          p[2] += a.p_c[j] * cj;
+          p[2] += (double)pc[j] * cj;
         });
         break;
       }
       case OP_PERMUTE_OUT: {  // approx/approx.c:693-694
-        const double* C = VN(s.op_r0);
-        each(a.n, [&](uint32_t i) { a.x_io[i] = C[a.col_ridx[i]]; });
-        if (fl & F_CX) each(a.n, [&](uint32_t j) { p[0] += a.p_c[j] * C[j]; });
+        const VI* C = VN(s.op_r0);
+        each(a.n, [&](uint32_t i) { a.x_io[i] = (double)C[a.col_ridx[i]]; });
+        if (fl & F_CX)
+          each(a.n, [&](uint32_t j) { p[0] += (double)pc[j] * (double)C[j]; });
         break;
       }
       case OP_MPG_GRAD0: {  // approx/approx.c:538-539
-        double* G1 = VN(NG);
-        double* G2 = VN(NG2);
+        VI* G1 = VN(NG);
+        VI* G2 = VN(NG2);
         spmv_rows<1>(a.AT, VM(MXV), nullptr, [&](uint32_t j, double gx, double) {
-          const double gg = gx + a.p_c[j];
+          const VI gg = (VI)(gx + (double)pc[j]);
           G1[j] = gg;
           G2[j] = gg;
         });
         break;
       }
       case OP_MPG_GRAD: {  // approx/mpg.inc:161-174
-        const double* X = VN(NX);
-        const double* Z = VN(NZ);
-        double* G1 = VN(NG);
-        const double* G2 = VN(NG2);
+        const VI* X = VN(NX);
+        const VI* Z = VN(NZ);
+        VI* G1 = VN(NG);
+        const VI* G2 = VN(NG2);
         spmv_rows<1>(a.AT, VM(MXV), nullptr, [&](uint32_t j, double gx, double) {
-          const double gg = gx + a.p_c[j];
-          G1[j] = gg;
-          const double xj = X[j];
-          double xp = xj - gg;
-          xp = maxpd(a.p_l[j], xp);
-          xp = minpd(a.p_u[j], xp);
-          const double d = xj - xp;
+          const VI ggv = (VI)(gx + (double)pc[j]);
+          const double gg = (double)ggv;
+          G1[j] = ggv;
+          const double xj = (double)X[j];
+          const double d = xj - clamp_lu(j, xj - gg);
           p[0] += d * d;
-          const double yj = xj - Z[j];
-          const double uj = gg - G2[j];
+          const double yj = xj - (double)Z[j];
+          const double uj = gg - (double)G2[j];
           p[1] += yj * yj;
           p[2] += yj * uj;
         });
         break;
       }
       case OP_MPG_ZUPD: {  // approx/mpg.inc:180-182
-        const double* X = VN(NX);
-        double* Z = VN(NZ);
-        const double* G1 = VN(NG);
+        const VI* X = VN(NX);
+        VI* Z = VN(NZ);
+        const VI* G1 = VN(NG);
         const double st = s.op_a;
         each(a.n, [&](uint32_t j) {
-          const double t = X[j] - st * G1[j];
-          const double lo = maxpd(a.p_l[j], t);
-          Z[j] = minpd(lo, a.p_u[j]);
+          const double t = (double)X[j] - st * (double)G1[j];
+          const double lo = maxpd((double)P(a.p_l)[j], t);
+          Z[j] = (VI)minpd(lo, (double)P(a.p_u)[j]);
         });
         break;
       }
@@ -525,31 +580,38 @@ struct Ops {
         break;
       }
       case OP_PENALISE: {  // alm.c:106-119
-        spmv_rows<1>(a.AT0, a.lam, nullptr, [&](uint32_t j, double atl, double) {
+        spmv_rows64<1>(a.AT0, a.lam, nullptr, [&](uint32_t j, double atl, double) {
           a.linc[j] = a.alm_c[j] - atl;
         });
         each(a.m, [&](uint32_t i) { p[0] += a.lam[i] * a.rhs[i]; });
         break;
       }
       case OP_PERMUTE_PROBLEM: {  // approx/approx.c:192-211
-        double* wb = VM(MSX);
+        VI* wb = VM(MSX);
+        VC* prhs = P(a.p_rhs);
+        VC* pw = P(a.p_w);
         each(a.m, [&](uint32_t i) {
           const uint32_t src = a.row_idx[i];
-          a.p_rhs[i] = a.rhs[src];
-          const double w = a.weight[src];
-          a.p_w[i] = w;
-          wb[i] = w * (double)a.beta[i];
+          prhs[i] = (VC)a.rhs[src];
+          const VC w = (VC)a.weight[src];
+          pw[i] = w;
+          wb[i] = (VI)((double)w * (double)a.beta[i]);
         });
+        VC* pcw = P(a.p_c);
+        VC* pl = P(a.p_l);
+        VC* pu = P(a.p_u);
         each(a.n, [&](uint32_t j) {
           const uint32_t src = a.col_idx[j];
-          a.p_c[j] = a.linc[src];
-          a.p_l[j] = a.lower[src];
-          a.p_u[j] = a.upper[src];
+          pcw[j] = (VC)a.linc[src];
+          pl[j] = (VC)a.lower[src];
+          pu[j] = (VC)a.upper[src];
         });
         break;
       }
       case OP_STEP_SIZES: {  // approx/approx.c:135-186
-        const double* wb = VM(MSX);
+        const VI* wb = VM(MSX);
+        VC* pv = P(a.p_v);
+        VC* piv = P(a.p_iv);
         const Sell& M = a.AT;
         for (uint32_t sl = g.gwave; sl < M.nslices; sl += g.nwaves) {
           const uint64_t base = M.off[sl];
@@ -557,20 +619,20 @@ struct Ops {
           double acc = 0.0;
           for (uint32_t k = 0; k < len; k++) {
             const uint64_t e = base + (uint64_t)k * 64 + g.lane;
-            const double val = M.val[e];
-            acc = acc + (wb[M.col[e]] * val) * val;
+            const double val = mval<MT>(M, e);
+            acc = acc + ((double)wb[M.col[e]] * val) * val;
           }
           const uint32_t j = sl * 64 + g.lane;
           if (j < M.nrows) {
-            a.p_v[j] = acc;
-            a.p_iv[j] = (acc == 0 && a.p_c[j] == 0) ? 1.0 : 1.0 / acc;
+            pv[j] = (VC)acc;
+            piv[j] = (VC)((acc == 0 && (double)pc[j] == 0) ? 1.0 : 1.0 / acc);
           }
         }
         break;
       }
       case OP_ALM_VIOL: {  // alm.c:121-132, :134-169
         p[1] = -HUGE_VAL;
-        spmv_rows<1>(a.A0, a.x_io, nullptr, [&](uint32_t i, double ax, double) {
+        spmv_rows64<1>(a.A0, a.x_io, nullptr, [&](uint32_t i, double ax, double) {
           const double ri = ax - a.rhs[i];
           a.r[i] = ri;
           p[0] += ri * ri;
@@ -1050,7 +1112,10 @@ struct Ctl {
         s.niter = 4294967295ull;  // -1u (alm.c:255)
         s.max_pg = fmin(1.0, s.precision);
         s.max_value = -HUGE_VAL;
-        s.min_delta = s.inner_min_delta;
+        // With FP32 iterates the checkpoint delta cannot fall below a few
+        // FP32 ulps, so a stall is declared there instead of at 1e-11.
+        s.min_delta = s.mixed ? fmax(s.inner_min_delta, kMixedMinDelta)
+                              : s.inner_min_delta;
         s.period = 10000;
         s.pc = PC_A_INIT;
         break;

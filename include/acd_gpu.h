@@ -55,8 +55,14 @@ uint64_t acd_matrix_stored_entries(const sparse_matrix_t *, int transpose);
 /* ---- solver options ---- */
 enum acd_precision {
         ACD_FP64 = 0,    /* FP64 everywhere: parity form */
-        ACD_MIXED = 1    /* FP32 matrix values and x/z/u vectors, FP64 row
-                            accumulation, duals, norms and scalars */
+        ACD_MIXED = 1,   /* matrix values in BF16 when every value is exact
+                            there, else FP32; rhs, weights, bounds, costs and
+                            step sizes of the working copy in FP32; iterates,
+                            row accumulation, duals, norms and scalars FP64 */
+        ACD_FP32 = 2     /* ACD_MIXED plus FP32 iterates (x, z, gradients,
+                            violation vectors): fewest bytes, but the FP32
+                            rounding of the iterates limits the reachable
+                            KKT residual (about 1e-5..1e-3 relative) */
 };
 
 struct acd_alm_options {

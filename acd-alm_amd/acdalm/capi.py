@@ -182,6 +182,8 @@ class CLib:
             ("acd_spmv_set_variant", C.c_int, [C.c_int]),
             ("acd_last_solve_profile", None, [C.POINTER(C.c_uint64),
                                               C.POINTER(C.c_double), C.c_int]),
+            ("acd_last_solve_bytes", None, [C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double), C.c_int]),
             ("acd_pass_name", C.c_char_p, [C.c_int]),
         ]:
             fn = getattr(L, name, None)
@@ -340,6 +342,21 @@ class CLib:
         for i in range(32):
             if cnt[i]:
                 out[self.lib.acd_pass_name(i).decode()] = (int(cnt[i]), float(us[i]))
+        return out
+
+    def profile_bytes(self):
+        """{pass name: (launches, step microseconds, algorithmic bytes)}."""
+        cnt = (C.c_uint64 * 32)()
+        us = (C.c_double * 32)()
+        sus = (C.c_double * 32)()
+        by = (C.c_double * 32)()
+        self.lib.acd_last_solve_profile(cnt, us, 32)
+        self.lib.acd_last_solve_bytes(sus, by, 32)
+        out = {}
+        for i in range(32):
+            if cnt[i]:
+                out[self.lib.acd_pass_name(i).decode()] = (
+                    int(cnt[i]), float(sus[i]), float(by[i]))
         return out
 
     def stats(self):

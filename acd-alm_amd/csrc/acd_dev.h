@@ -181,6 +181,10 @@ int acd_dev_solve(struct acd_dev_lp *, const struct acd_solve_request *,
 /* Per-op launch counts and microseconds (step + control) of the last
  * solve, indexed by op code. */
 void acd_dev_last_profile(uint64_t * counts, double * us, int n);
+/* Per op of the last solve: microseconds from the step kernel's start to
+ * the control kernel's start (the step kernel's duration plus the control
+ * launch latency), and algorithmic bytes moved. */
+void acd_dev_last_profile_ex(double * step_us, double * bytes, int n);
 
 #ifdef __cplusplus
 }

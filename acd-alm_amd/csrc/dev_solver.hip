@@ -35,25 +35,27 @@ enum {
 };
 uint64_t g_prof_count[32];
 double g_prof_us[32];
+double g_prof_step_us[32];
+double g_prof_bytes[32];
 
 struct GridG {
   uint32_t lane, gwave, nwaves;
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs; renumber them so
-// that each XCD (and its L2) walks a contiguous range of slices.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
-  if (nb % 8) return b;
-  return (b % 8) * (nb / 8) + b / 8;
-}
-
+// Slice s goes to wave s (grid-stride beyond the grid).  Workgroups are
+// dispatched round-robin over the 8 XCDs, so consecutive slices land on
+// different XCDs and an operator with fewer slices than the grid has waves
+// (A of a set-cover LP: 3,125 slices for 8,192 waves) still spreads over
+// every XCD.  Renumbering blocks into contiguous per-XCD ranges put all of
+// such an operator on the first three XCDs and halved its throughput.
 template <class VI, class VC, class MT>
 __global__ void __launch_bounds__(64 * kWPB) k_step(Args a) {
   const State& s = *a.st;
   if (s.op == OP_NONE) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.st->t_step = wall_clock64();
   GridG g;
   g.lane = threadIdx.x & 63;
-  g.gwave = xcd_block(blockIdx.x, gridDim.x) * kWPB + (threadIdx.x >> 6);
+  g.gwave = blockIdx.x * kWPB + (threadIdx.x >> 6);
   g.nwaves = gridDim.x * kWPB;
   Ops<GridG, VI, VC, MT> ops(g, a, s);
   ops.run();
@@ -79,9 +81,17 @@ __global__ void __launch_bounds__(64 * kWPB) k_step(Args a) {
 // by butterfly), then advances the state machine on one thread.
 __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
   __shared__ double res[kRes];
+  __shared__ State ls;
   State* st = a.st;
-  const uint32_t mm = st->op_maxmask;
-  const bool ran = st->op != OP_NONE;
+  {  // the state record moves through LDS, copied by the whole workgroup
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(st);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(&ls);
+    for (uint32_t i = threadIdx.x; i < sizeof(State) / 8; i += 256)
+      dst[i] = src[i];
+  }
+  __syncthreads();
+  const uint32_t mm = ls.op_maxmask;
+  const bool ran = ls.op != OP_NONE;
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int kk = 0; kk < 2; kk++) {
@@ -101,18 +111,27 @@ __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
     if (lane == 0) res[k] = acc;
   }
   __syncthreads();
-  if (threadIdx.x == 0 && st->pc != PC_DONE && !st->error) {
+  const bool live = ls.pc != PC_DONE && !ls.error;
+  if (threadIdx.x == 0 && live) {
     const uint64_t now = wall_clock64();
-    const int32_t op = st->op;
+    const int32_t op = ls.op;
     if (ran && op >= 0 && op < 32) {
-      st->op_count[op]++;
-      st->op_ticks[op] += now - st->t_last;
+      Prof* pf = a.prof;
+      pf->op_count[op]++;
+      pf->op_ticks[op] += now - ls.t_last;
+      pf->step_ticks[op] += now - ls.t_step;
+      pf->op_bytes[op] += alg_bytes(op, ls.op_flags, a);
     }
-    State s = *st;
-    Ctl c{s, a, res};
+    Ctl c{ls, a, res};
     c.advance();
-    s.t_last = wall_clock64();
-    *st = s;
+    ls.t_last = wall_clock64();
+  }
+  __syncthreads();
+  if (live) {
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(&ls);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(st);
+    for (uint32_t i = threadIdx.x; i < sizeof(State) / 8; i += 256)
+      dst[i] = src[i];
   }
 }
 
@@ -140,6 +159,7 @@ struct acd_dev_lp {
   double* vn[kNumVN];
   double* vm[kNumVM];
   State* st;
+  Prof* prof;
   double* partials;
   acd_event* ev;
   int blocks;
@@ -150,7 +170,9 @@ struct acd_dev_lp {
 
 namespace {
 
-Args make_args(acd_dev_lp* lp) {
+Args make_args(acd_dev_lp* lp, int key) {
+  static const uint32_t kSizes[kKeys][3] = {
+      {8, 8, 8}, {8, 4, 4}, {8, 4, 2}, {4, 4, 4}, {4, 4, 2}};
   Args a;
   memset(&a, 0, sizeof a);
   a.m = lp->m;
@@ -182,9 +204,14 @@ Args make_args(acd_dev_lp* lp) {
   a.p_v = lp->p_v;
   a.p_iv = lp->p_iv;
   a.st = lp->st;
+  a.prof = lp->prof;
   a.partials = lp->partials;
   a.ev = lp->ev;
   a.ev_cap = kEventCap;
+  a.nnz = lp->orig->nnz;
+  a.si = kSizes[key][0];
+  a.sc = kSizes[key][1];
+  a.sv = kSizes[key][2];
   return a;
 }
 
@@ -224,7 +251,7 @@ int build_graph(acd_dev_lp* lp, int key) {
   hipStream_t cs;
   if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
     return -1;
-  Args a = make_args(lp);
+  Args a = make_args(lp, key);
   hipGraph_t g;
   int rc = 0;
   if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) !=
@@ -284,7 +311,7 @@ void init_state(State* s, const acd_dev_lp* lp, const acd_solve_request* q) {
 int run_to_completion(acd_dev_lp* lp, int key, State* hs,
                       acd_solve_result* res, acd_event_sink sink, void* ctx) {
   if (build_graph(lp, key)) return -1;
-  Args a = make_args(lp);
+  Args a = make_args(lp, key);
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
@@ -292,7 +319,8 @@ int run_to_completion(acd_dev_lp* lp, int key, State* hs,
   int rc = 0;
   uint64_t launches = 0;
   if (hipMemcpyAsync(lp->st, hs, sizeof(State), hipMemcpyHostToDevice, S()) !=
-      hipSuccess)
+          hipSuccess ||
+      hipMemsetAsync(lp->prof, 0, sizeof(Prof), S()) != hipSuccess)
     rc = -1;
   hipEventRecord(e0, S());
   if (!rc) k_ctl<<<1, 256, 0, S()>>>(a, (uint32_t)lp->blocks);
@@ -336,12 +364,18 @@ int run_to_completion(acd_dev_lp* lp, int key, State* hs,
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   {
+    Prof pf;
+    memset(&pf, 0, sizeof pf);
+    hipMemcpy(&pf, lp->prof, sizeof pf, hipMemcpyDeviceToHost);
     int khz = 100000;
     hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate,
                           acd_dev_device());
     for (int i = 0; i < 32; i++) {
-      g_prof_count[i] = hs->op_count[i];
-      g_prof_us[i] = khz > 0 ? 1e3 * (double)hs->op_ticks[i] / khz : 0.0;
+      g_prof_count[i] = pf.op_count[i];
+      g_prof_us[i] = khz > 0 ? 1e3 * (double)pf.op_ticks[i] / khz : 0.0;
+      g_prof_step_us[i] =
+          khz > 0 ? 1e3 * (double)pf.step_ticks[i] / khz : 0.0;
+      g_prof_bytes[i] = pf.op_bytes[i];
     }
   }
   if (res) {
@@ -368,12 +402,17 @@ int run_to_completion(acd_dev_lp* lp, int key, State* hs,
 int precision_key(acd_dev_lp* lp, int precision) {
   if (precision != ACD_DEV_MIXED && precision != ACD_DEV_FP32) return K_F64;
   acd_dev_matrix* pm = const_cast<acd_dev_matrix*>(lp->perm);
-  const int ba = acd_dev_sell_make_bf16(&pm->a);
-  const int bt = acd_dev_sell_make_bf16(&pm->at);
-  if (ba < 0 || bt < 0) return -1;
-  const bool bf = ba == 1 && bt == 1;
-  if (!bf && (acd_dev_sell_make_fp32(&pm->a) || acd_dev_sell_make_fp32(&pm->at)))
-    return -1;
+  acd_dev_matrix* om = const_cast<acd_dev_matrix*>(lp->orig);
+  acd_sell* ops[4] = {&pm->a, &pm->at, &om->a, &om->at};
+  bool bf = true;
+  for (acd_sell* op : ops) {
+    const int r = acd_dev_sell_make_bf16(op);
+    if (r < 0) return -1;
+    bf = bf && r == 1;
+  }
+  if (!bf)
+    for (acd_sell* op : ops)
+      if (acd_dev_sell_make_fp32(op)) return -1;
   if (precision == ACD_DEV_FP32) return bf ? K_F32_BF16 : K_F32;
   return bf ? K_MIX_BF16 : K_MIX_F32;
 }
@@ -386,6 +425,13 @@ void acd_dev_last_profile(uint64_t* counts, double* us, int n) {
   for (int i = 0; i < n && i < 32; i++) {
     counts[i] = g_prof_count[i];
     us[i] = g_prof_us[i];
+  }
+}
+
+void acd_dev_last_profile_ex(double* step_us, double* bytes, int n) {
+  for (int i = 0; i < n && i < 32; i++) {
+    step_us[i] = g_prof_step_us[i];
+    bytes[i] = g_prof_bytes[i];
   }
 }
 
@@ -420,8 +466,9 @@ struct acd_dev_lp* acd_dev_lp_create(const struct acd_dev_matrix* orig,
   lp->blocks = pick_blocks(orig, perm);
   lp->partials = al((size_t)lp->blocks * kRes);
   lp->st = (State*)acd_dev_malloc(sizeof(State));
+  lp->prof = (Prof*)acd_dev_malloc(sizeof(Prof));
   lp->ev = (acd_event*)acd_dev_malloc(sizeof(acd_event) * kEventCap);
-  if (!ok || !lp->st || !lp->ev) {
+  if (!ok || !lp->st || !lp->prof || !lp->ev) {
     acd_dev_lp_free(lp);
     return nullptr;
   }
@@ -445,6 +492,7 @@ void acd_dev_lp_free(struct acd_dev_lp* lp) {
   for (int i = 0; i < kNumVM; i++) acd_dev_free(lp->vm[i]);
   acd_dev_free(lp->partials);
   acd_dev_free(lp->st);
+  acd_dev_free(lp->prof);
   acd_dev_free(lp->ev);
   free(lp);
 }

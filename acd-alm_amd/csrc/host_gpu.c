@@ -62,6 +62,11 @@ void acd_last_solve_profile(uint64_t * counts, double * us, int n)
         acd_dev_last_profile(counts, us, n);
 }
 
+void acd_last_solve_bytes(double * step_us, double * bytes, int n)
+{
+        acd_dev_last_profile_ex(step_us, bytes, n);
+}
+
 const char * acd_pass_name(int p)
 {
         static const char * names[] = {

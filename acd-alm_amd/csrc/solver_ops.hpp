@@ -160,9 +160,17 @@ struct State {
   double scale, argmax;
   // bookkeeping
   uint64_t inner_total, events, ops;
+  uint64_t t_last;              // wall clock when the last op was issued
+  uint64_t t_step;              // wall clock at the step kernel's start
+};
+static_assert(sizeof(State) % 8 == 0, "State is copied as 64-bit words");
+
+// Per-op counters of one solve, kept beside the state.
+struct Prof {
   uint64_t op_count[32];        // launches per op
   uint64_t op_ticks[32];        // wall-clock ticks per op (step + control)
-  uint64_t t_last;              // wall clock when the last op was issued
+  uint64_t step_ticks[32];      // ticks from step kernel start to control
+  double op_bytes[32];          // algorithmic bytes moved per op
 };
 
 struct Args {
@@ -178,9 +186,12 @@ struct Args {
   // working-copy vectors
   double *p_rhs, *p_w, *p_c, *p_l, *p_u, *p_v, *p_iv;
   State* st;
+  Prof* prof;
   double* partials;             // [blocks][kRes]
   acd_event* ev;
   uint32_t ev_cap;
+  uint64_t nnz;                 // nonzeros of A
+  uint32_t si, sc, sv;          // bytes per iterate / constant / value
 };
 
 // SSE2 maxpd(a, b) / minpd(a, b) of the reference's projections
@@ -200,11 +211,16 @@ __device__ __forceinline__ double wave_reduce(double v, bool is_max) {
 
 // ---------------------------------------------------------------------------
 // Ops.  G supplies lane, gwave (this wave's index among the group's waves)
-// and nwaves; slice s of 64 rows goes to wave s mod nwaves.  V is the type
-// of the working-copy vectors (double, or float in the mixed form) and MT
-// that of the working-copy matrix values (double, float or bf16); every
-// product is formed and accumulated in double.  The ALM-level passes over
-// the caller's matrix and vectors stay in double.
+// and nwaves; slice s of 64 rows goes to wave s mod nwaves.  VI is the
+// type of the iterates (x, z, gradients, violation vectors), VC that of the
+// working copy's per-column constants (cost, bounds, step sizes) and MT that
+// of the stored matrix values (double, float or bf16 bits).  Row-side
+// constants (rhs, weights), multipliers and the caller's vectors stay in
+// double, and every product is formed and accumulated in double.  The ALM
+// residual and penalty passes read the caller's matrix in MT as well, so
+// that the outer loop drives the residual of the same (rounded) model the
+// inner solver minimises; a mismatch there would be absorbed into the
+// multipliers scaled by the row weights.
 template <class G, class VI = double, class VC = double, class MT = double>
 struct Ops {
   const G& g;
@@ -263,19 +279,14 @@ struct Ops {
       if (r < M.nrows) epi(r, a0, a1);
     }
   }
-  // Working-copy operators (A, A') in the solver precision.
+  // Working-copy operators (A, A') in the solver precision.  The ALM
+  // passes over the caller's operators (A0, A0') read the same stored
+  // precision so that the outer and inner problems agree.
   template <int NRHS, class E>
   __device__ __forceinline__ void spmv_rows(const Sell& M, const VI* x0,
                                             const VI* x1, E&& epi) {
     spmv_rows_t<NRHS, MT, VI>(M, x0, x1, epi);
   }
-  // Caller's operators (A0, A0') in double.
-  template <int NRHS, class E>
-  __device__ __forceinline__ void spmv_rows64(const Sell& M, const double* x0,
-                                              const double* x1, E&& epi) {
-    spmv_rows_t<NRHS, double, double>(M, x0, x1, epi);
-  }
-
   template <class F>
   __device__ __forceinline__ void each(uint32_t n, F&& f) {
     const uint32_t ns = (n + 63) >> 6;
@@ -319,8 +330,8 @@ struct Ops {
         VI* c1 = VM(MZV);
         VI* c2 = VM(MYV);
         const bool copy2 = fl & F_COPY2;
-        const VC* b = P(a.p_rhs);
-        const VC* w = P(a.p_w);
+        const double* b = a.p_rhs;
+        const double* w = a.p_w;
         spmv_rows<1>(a.A, X, nullptr, [&](uint32_t i, double ax, double) {
           const double v = ax - (double)b[i];
           const double sv = (double)w[i] * v;
@@ -394,8 +405,8 @@ struct Ops {
         VI* YV = VM(MYV);
         VI* SX = VM(MSX);
         VI* SY = VM(MSY);
-        const VC* b = P(a.p_rhs);
-        const VC* w = P(a.p_w);
+        const double* b = a.p_rhs;
+        const double* w = a.p_w;
         const bool spec = fl & F_SPEC;
         const double th = s.op_a, thn = s.op_b;
         const double sc = 1 - th, nsc = 1 - thn;
@@ -580,7 +591,7 @@ struct Ops {
         break;
       }
       case OP_PENALISE: {  // alm.c:106-119
-        spmv_rows64<1>(a.AT0, a.lam, nullptr, [&](uint32_t j, double atl, double) {
+        spmv_rows_t<1, MT, double>(a.AT0, a.lam, nullptr, [&](uint32_t j, double atl, double) {
           a.linc[j] = a.alm_c[j] - atl;
         });
         each(a.m, [&](uint32_t i) { p[0] += a.lam[i] * a.rhs[i]; });
@@ -588,12 +599,12 @@ struct Ops {
       }
       case OP_PERMUTE_PROBLEM: {  // approx/approx.c:192-211
         VI* wb = VM(MSX);
-        VC* prhs = P(a.p_rhs);
-        VC* pw = P(a.p_w);
+        double* prhs = a.p_rhs;
+        double* pw = a.p_w;
         each(a.m, [&](uint32_t i) {
           const uint32_t src = a.row_idx[i];
-          prhs[i] = (VC)a.rhs[src];
-          const VC w = (VC)a.weight[src];
+          prhs[i] = a.rhs[src];
+          const double w = a.weight[src];
           pw[i] = w;
           wb[i] = (VI)((double)w * (double)a.beta[i]);
         });
@@ -632,7 +643,7 @@ struct Ops {
       }
       case OP_ALM_VIOL: {  // alm.c:121-132, :134-169
         p[1] = -HUGE_VAL;
-        spmv_rows64<1>(a.A0, a.x_io, nullptr, [&](uint32_t i, double ax, double) {
+        spmv_rows_t<1, MT, double>(a.A0, a.x_io, nullptr, [&](uint32_t i, double ax, double) {
           const double ri = ax - a.rhs[i];
           a.r[i] = ri;
           p[0] += ri * ri;
@@ -671,6 +682,65 @@ struct Ops {
     }
   }
 };
+
+// ---------------------------------------------------------------------------
+// Algorithmic bytes of one op: each stored nonzero read once (value +
+// 32-bit column), the slice offsets, the gathered vector counted once per
+// element, and every streamed vector read or written once.  Padding and
+// repeated gathers are not counted: they are the kernel's overhead, which
+// the achieved fraction of the bandwidth roofline exposes.
+__device__ inline double alg_bytes(int32_t op, uint32_t fl, const Args& a) {
+  const double m = a.m, n = a.n, si = a.si, sc = a.sc;
+  const double mat = (double)a.nnz * (a.sv + 4.0);
+  const double offA = 8.0 * (a.A.nslices + 1), offT = 8.0 * (a.AT.nslices + 1);
+  const double spA = mat + offA, spT = mat + offT;  // A rows / A' rows
+  switch (op) {
+    case OP_INIT_X:
+      return n * (8 + 4 + 2 * sc + 2 * si) +
+             ((fl & F_APPROX) ? n * (si + sc) : 0);
+    case OP_VIOL:
+      return spA + n * si + m * (16 + si) + ((fl & F_COPY2) ? 2 * m * si : 0) +
+             ((fl & F_CX) ? n * (si + sc) : 0);
+    case OP_PASS1: {
+      const bool two = fl & F_TWO;
+      double b = spT + m * si * (two ? 2 : 1);
+      b += n * (4 * sc + 3 * si);  // c, iv, l, u; z, g2, zp
+      if (fl & F_LONG) b += n * sc;
+      if (two) b += n * si;
+      if (fl & (F_PCOPY | F_LAZY)) b += n * si;
+      if (fl & F_PCOPY) b += n * si;
+      if (fl & F_LAZY) b += n * si;
+      return b;
+    }
+    case OP_PASS2:
+      return spA + n * si + m * (16 + 4 * si);
+    case OP_FIXUP: {
+      double b = n * si;  // z
+      b += (fl & F_NEEDG) ? spT + m * si + n * (sc + si) : n * si;
+      b += (fl & F_RESTEP) ? n * (si + 3 * sc + si) : n * si;
+      return b;
+    }
+    case OP_RESTART_COPY: return 2 * n * si + 3 * m * si;
+    case OP_CKPT_X:
+      return n * (2 * si + sc) + ((fl & F_LAZY) ? 2 * n * si : 0);
+    case OP_GRAD_PG: return spT + m * si + n * (3 * sc + 2 * si);
+    case OP_NORM_G: return n * si;
+    case OP_FIN_A: return n * (2 * si + sc);
+    case OP_PERMUTE_OUT:
+      return n * (si + 4 + 8) + ((fl & F_CX) ? n * (si + sc) : 0);
+    case OP_MPG_GRAD0: return spT + m * si + n * (sc + 2 * si);
+    case OP_MPG_GRAD: return spT + m * si + n * (3 * sc + 4 * si);
+    case OP_MPG_ZUPD: return n * (3 * si + 2 * sc);
+    case OP_ALM_LAMPROJ: return 32 * m + 16 * n;
+    case OP_PENALISE: return spT + 8 * m + 16 * n + 16 * m;
+    case OP_PERMUTE_PROBLEM: return m * (40 + si) + n * (28 + 3 * sc);
+    case OP_STEP_SIZES: return spT + m * si + 3 * n * sc;
+    case OP_ALM_VIOL: return spA + 8 * n + 24 * m + 16 * n;
+    case OP_ARGMAX: return 8 * m;
+    case OP_LAM_W: return 48 * m + ((fl & F_UPD) ? 8 * m : 0);
+    default: return 0;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Control: consumes the reduced results of the op just run, advances.

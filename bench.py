@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""bench.py -- LP solve throughput of the device-resident ALM + APPROX path
+on one MI355X.
+
+A step is one complete ``alm_solve`` (ALM outer loop, APPROX inner loop,
+SpMV) of the set-cover LP relaxation (m = 200,000 rows, n = 1,000,000
+columns plus m slack columns, 0/1 coefficients, 3-8 nonzeros per column,
+c ~ U[1,100], 0 <= x <= 1; seeded synthetic instance), from x = 0 and
+lambda = 0 with the reference's stopping rule (alm.c:303-319).  The matrix,
+the problem vectors and all solver state are resident in HBM before the
+timed region; each step restores the multiplier weights and penalised cost
+(acd_alm_reset) and solves again.
+
+Printed (one JSON line): solves/s over the K timed steps, p50/p99 latency
+per solve, the relative KKT residual of the result (checked <= 1e-6), the
+roofline of the solver's step kernel (algorithmic bytes / kernel time,
+both accumulated by the solver's own control kernel over the timed solves),
+the stand-alone SpMV kernel's HBM throughput on the same matrix (HIP
+events), and the CPU baseline: the reference's thread_pool CPU path
+(oracle/_ref/ref_bench, built from the reference sources) or, where that
+build is absent, the oracle's C restatement, timed on a bounded sample of
+APPROX iterations of the same instance and scaled by the number of inner
+iterations the solve takes.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+                       [--precision mixed|fp64] [--cpu-iters N] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(HERE, "acd-alm_amd"), os.path.join(HERE, "oracle")]
+
+METRIC = ("solves/s + p50/p99 latency at 1e-6 rel KKT residual; "
+          "SpMV HBM GB/s vs peak")
+HBM_PEAK_GBS = 8000.0
+KKT_TOL = 1e-6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_leg(lib, mat, lp, reps=50):
+    """Stand-alone SpMV (y = A x and y = A' x) on the solver's matrix,
+    timed with HIP events on the library stream."""
+    L = lib.lib
+    out = {}
+    for tr in (0, 1):
+        ny, nx = (lp.ncols, lp.nrows) if tr else (lp.nrows, lp.ncols)
+        dx = L.acd_device_alloc(8 * nx)
+        dy = L.acd_device_alloc(8 * ny)
+        x = np.random.default_rng(7).random(nx)
+        L.acd_copy_to_device(dx, x.ctypes.data, 8 * nx)
+        ms = C.c_double()
+        rc = L.acd_spmv_device_time(mat, tr, dy, dx, reps, C.byref(ms))
+        L.acd_device_free(dx)
+        L.acd_device_free(dy)
+        if rc:
+            raise RuntimeError("SpMV timing failed: %s" % (lib.last_error(),))
+        slices = (ny + 63) // 64
+        alg = lp.nnz * (8 + 4) + 8 * (slices + 1) + 8 * nx + 8 * ny
+        gbs = alg / (ms.value * 1e-3) / 1e9
+        out["AT" if tr else "A"] = {
+            "us": round(ms.value * 1e3, 2), "alg_bytes": int(alg),
+            "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def cpu_baseline(lp, inner_iterations, iters, threads):
+    """Bounded CPU sample: `iters` APPROX iterations of the first ALM
+    subproblem on the reference build (or the oracle restatement), scaled to
+    the inner iterations one solve takes."""
+    ref = os.path.join(HERE, "oracle", "_ref", "ref_bench")
+    with tempfile.TemporaryDirectory() as td:
+        if os.path.exists(ref):
+            path = os.path.join(td, "lp.lpb")
+            lp.write_binary(path)
+            r = subprocess.run([ref, path, str(threads), "approx", str(iters)],
+                               capture_output=True, text=True, timeout=900)
+            if r.returncode:
+                raise RuntimeError("ref_bench failed: " + r.stderr[-400:])
+            res = json.loads(r.stdout.strip().splitlines()[-1])
+            spi = res["s_per_iteration"]
+            kind, cores = "reference", threads
+        else:
+            from pyoracle import Oracle
+            ora = Oracle()
+            m = ora.make_matrix(lp, 0)
+            a = ora.approx_make(m, lp)
+            ora.approx_update(a)
+            t = time.perf_counter()
+            ora.approx_solve(a, np.zeros(lp.ncols), iters)
+            spi = (time.perf_counter() - t) / iters
+            ora.approx_free(a)
+            ora.free_matrix(m)
+            kind, cores = "port", 1
+    per_solve = spi * inner_iterations
+    return {"value": 1.0 / per_solve, "unit": "solves/s", "cores": cores,
+            "kind": kind,
+            "sample": ("%d APPROX iterations of the first ALM subproblem of "
+                       "the same instance (%.4g s/iteration on %d thread%s), "
+                       "scaled by the %d inner iterations of one solve"
+                       % (iters, spi, cores, "" if cores == 1 else "s",
+                          inner_iterations))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--cpu-iters", type=int, default=300)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--outer", type=int, default=1000,
+                    help="ALM outer-iteration cap (the reference uses 1000)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU "
+              "path (each LP solve runs on one device).", file=sys.stderr)
+        sys.exit(2)
+
+    from acdalm import capi, instances, kkt
+    lib = capi.load()
+    L = lib.lib
+    prec = {"mixed": 1, "fp64": 0}[args.precision]
+
+    t0 = time.perf_counter()
+    lp = instances.set_cover()
+    mat = lib.make_matrix(lp, 0)
+    alm = lib.alm_make(mat, lp)
+    x0 = np.zeros(lp.ncols)
+    lam0 = np.zeros(lp.nrows)
+    # Untimed priming: one outer iteration builds the solver graphs and the
+    # reduced-precision copies of the matrix.
+    lib.alm_solve_ex(alm, 1, x0, lam0, precision=prec)
+    setup_s = time.perf_counter() - t0
+    log("setup %.1f s (m=%d n=%d nnz=%d)" % (setup_s, lp.nrows, lp.ncols, lp.nnz))
+
+    def one_solve():
+        L.acd_alm_reset(alm)
+        t = time.perf_counter()
+        rc, x, lam, diag, _ = lib.alm_solve_ex(alm, args.outer, x0, lam0,
+                                               precision=prec)
+        return time.perf_counter() - t, rc, x, lam, diag
+
+    for w in range(args.warmup):
+        dt, rc, *_ = one_solve()
+        log("warmup %d: %.2f s rc=%d" % (w, dt, rc))
+
+    L.acd_synchronize()
+    lat = []
+    step_us = np.zeros(32)
+    bytes_ = np.zeros(32)
+    launches = 0
+    inner = outer = 0
+    device_ms = 0.0
+    objs = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        dt, rc, x, lam, diag = one_solve()
+        lat.append(dt)
+        st = lib.stats()
+        inner, outer = st.inner_iterations, st.outer_iterations
+        device_ms += st.device_ms
+        for name, (cnt, sus, by) in lib.profile_bytes().items():
+            i = [L.acd_pass_name(j).decode() for j in range(32)].index(name)
+            step_us[i] += sus
+            bytes_[i] += by
+            launches += cnt
+        objs.append(diag[0])
+        log("step %d: %.2f s rc=%d inner=%d outer=%d obj=%.12g"
+            % (k, dt, rc, inner, outer, diag[0]))
+    L.acd_synchronize()
+    total = time.perf_counter() - t_start
+
+    res = kkt.kkt(lp, x, lam)
+    kkt_ok = bool(rc == 0 and res["kkt"] <= KKT_TOL)
+    lat_ms = np.array(lat) * 1e3
+    spmv = spmv_leg(lib, mat, lp)
+
+    achieved = bytes_.sum() / (step_us.sum() * 1e-6) / 1e9
+    names = [L.acd_pass_name(j).decode() for j in range(32)]
+    top = np.argsort(-step_us)[:3]
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "kernel": "k_step (solver step kernel, all passes)",
+        "launches": int(launches),
+        "avg_us": round(step_us.sum() / max(launches, 1), 2),
+        "alg_bytes_per_launch": round(bytes_.sum() / max(launches, 1)),
+        "by_pass": {names[i]: {
+            "GB/s": round(bytes_[i] / (step_us[i] * 1e-6) / 1e9, 1),
+            "share": round(step_us[i] / step_us.sum(), 3)}
+            for i in top if step_us[i] > 0},
+    }
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            cpu = cpu_baseline(lp, inner, args.cpu_iters, threads)
+        except Exception as e:  # reported, never silently replaced
+            cpu = {"value": None, "error": str(e)[-300:]}
+
+    line = {
+        "metric": METRIC,
+        "value": args.steps / total,
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded set-cover generator)",
+        "config": {"workload": "set-cover LP relaxation m=200000 n=1000000"
+                               "+200000 slack, 0/1 A, 3-8 nnz/column",
+                   "precision": args.precision,
+                   "nnz": int(lp.nnz), "m": int(lp.nrows), "n": int(lp.ncols),
+                   "parallelism": "single GPU"},
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "kkt": res["kkt"], "kkt_ok": kkt_ok,
+        "objective": objs[-1], "deterministic": len(set(objs)) == 1,
+        "inner_iterations": int(inner), "outer_iterations": int(outer),
+        "solver_device_ms": device_ms / args.steps,
+        "roofline": roofline,
+        "spmv": spmv,
+        "cpu_baseline": cpu,
+        "setup_s": setup_s,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

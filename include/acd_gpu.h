@@ -95,6 +95,12 @@ void acd_last_solve_stats(struct acd_solve_stats *);
  * of launches and the microseconds from its issue to the next decision.
  * Pass names are listed by acd_pass_name(). */
 void acd_last_solve_profile(uint64_t * counts, double * us, int n);
+/* For pass p < n of the last solve: microseconds from the start of its
+ * step kernels to the start of the following control kernels (kernel
+ * duration plus one launch latency), and the algorithmic bytes those
+ * launches moved (each stored nonzero, slice offset and vector element
+ * read or written once). */
+void acd_last_solve_bytes(double * step_us, double * bytes, int n);
 const char * acd_pass_name(int p);
 
 #ifdef __cplusplus

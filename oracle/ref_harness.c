@@ -9,6 +9,9 @@
  * usage: ref_bench INSTANCE.lpb THREADS MODE [REPEAT]
  *   MODE alm  : alm_solve(niter=1000) from x = 0, lambda = 0
  *   MODE spmv : REPEAT x (A x, then A' y)
+ *   MODE approx : REPEAT APPROX iterations of the first ALM subproblem
+ *                 (lambda = 0, unit weights, the instance's cost), from
+ *                 x = 0, with no stopping test; the bounded CPU sample
  * Prints one JSON line.  The instance format (little endian):
  *   "ACDLP001", u64 nrows, u64 ncolumns, u64 nnz,
  *   u32 rows[nnz], u32 columns[nnz], f64 values[nnz],
@@ -21,7 +24,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <math.h>
 #include "alm.h"
+#include "approx/approx.h"
 
 static double now_s(void)
 {
@@ -95,6 +100,26 @@ int main(int argc, char ** argv)
                        "\"make_s\": %.6g, \"checksum\": %.17g}\n",
                        threads, repeat, ta / repeat, tt / repeat, t_make,
                        ox[0] + oy[0]);
+        } else if (!strcmp(mode, "approx")) {
+                double * w = calloc(m + 2, 8), * x = calloc(n + 2, 8);
+                for (size_t i = 0; i < m; i++) w[i] = 1;
+                approx_t * ap = approx_make(a, m, rhs, w, n, lin, lo, hi);
+                double s = now_s();
+                approx_update(ap);
+                double t_update = now_s() - s;
+                double diag[5];
+                s = now_s();
+                int ret = approx_solve(x, n, ap, (size_t)repeat, 0.0,
+                                       -HUGE_VAL, 0.0, NULL, 0, diag, 0.0,
+                                       pool);
+                double t = now_s() - s;
+                printf("{\"mode\": \"approx\", \"threads\": %d, "
+                       "\"ret\": %d, \"iterations\": %d, "
+                       "\"s_per_iteration\": %.9g, \"update_s\": %.6g, "
+                       "\"make_s\": %.6g, \"objective\": %.17g}\n",
+                       threads, ret, repeat, t / repeat, t_update, t_make,
+                       diag[0]);
+                approx_free(ap);
         } else {
                 alm_t * alm = alm_make(a, m, rhs, n, lin, lo, hi, llo, lhi);
                 double * x = calloc(n + 2, 8), * lam = calloc(m + 2, 8);

@@ -167,6 +167,7 @@ class CLib:
             ("acd_device_alloc", vp, [sz]),
             ("acd_device_free", None, [vp]),
             ("acd_copy_to_device", C.c_int, [vp, vp, sz]),
+            ("acd_copy_device", C.c_int, [vp, vp, sz]),
             ("acd_copy_to_host", C.c_int, [vp, vp, sz]),
             ("acd_spmv_device", C.c_int, [vp, C.c_int, vp, vp]),
             ("acd_spmv_device_time", C.c_int, [vp, C.c_int, vp, vp, C.c_int,

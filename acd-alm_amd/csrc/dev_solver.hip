@@ -24,6 +24,7 @@ inline hipStream_t S() { return (hipStream_t)acd_dev_stream(); }
 constexpr int kWPB = 4;  // wavefronts per workgroup of the step kernel
 constexpr uint32_t kEventCap = 4096;
 constexpr int kPairs = 32;  // (step, control) pairs per graph
+constexpr uint32_t kMaxBlocks = 2048;  // step-kernel grid cap (control loads)
 // Precision keys: iterate vectors / constant vectors / matrix values.
 enum {
   K_F64 = 0,    // double / double / double
@@ -72,18 +73,34 @@ __global__ void __launch_bounds__(64 * kWPB) k_step(Args a) {
     const bool mx = (mm >> k) & 1;
     double acc = lds[0][k];
     for (int w = 1; w < kWPB; w++) acc = combine(acc, lds[w][k], mx);
-    a.partials[(size_t)blockIdx.x * kRes + k] = acc;
+    a.partials[(size_t)k * gridDim.x + blockIdx.x] = acc;
   }
 }
 
-// Combines the step kernel's per-block partials (wave w owns slots 2w and
-// 2w+1; lane l sums blocks l, l+64, ... in order, then the lanes combine
-// by butterfly), then advances the state machine on one thread.
+// Combines the step kernel's per-block partials (slot-major: slot k of
+// block b at k * nblocks + b; wave w owns slots 2w and 2w+1; lane l sums
+// blocks l, l+64, ... in that order, then the lanes combine by butterfly),
+// then advances the state machine on one thread with the state record held
+// in LDS.  Every load (state words and all partials) is issued before the
+// first wait, so the kernel pays one memory latency instead of one per
+// batch.
 __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
   __shared__ double res[kRes];
   __shared__ State ls;
+  const uint64_t t_entry = wall_clock64();
   State* st = a.st;
-  {  // the state record moves through LDS, copied by the whole workgroup
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int kPer = kMaxBlocks / 64;
+  double v0[kPer], v1[kPer];
+  {
+    const double* p0 = a.partials + (size_t)(2 * w) * nblocks;
+    const double* p1 = p0 + nblocks;
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const uint32_t b = lane + 64 * j;
+      v0[j] = b < nblocks ? p0[b] : 0.0;
+      v1[j] = b < nblocks ? p1[b] : 0.0;
+    }
     const uint64_t* src = reinterpret_cast<const uint64_t*>(st);
     uint64_t* dst = reinterpret_cast<uint64_t*>(&ls);
     for (uint32_t i = threadIdx.x; i < sizeof(State) / 8; i += 256)
@@ -91,37 +108,37 @@ __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
   }
   __syncthreads();
   const uint32_t mm = ls.op_maxmask;
-  const bool ran = ls.op != OP_NONE;
-  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int32_t op = ls.op;
+  const bool ran = op != OP_NONE;
+  const bool live = ls.pc != PC_DONE && !ls.error;
+  // Per-op counters: lanes 0..3 each update one, in parallel.
+  if (threadIdx.x < 4 && live && ran && op >= 0 && op < 32) {
+    uint64_t d;
+    switch (threadIdx.x) {
+      case 0: d = 1; break;
+      case 1: d = t_entry - ls.t_last; break;
+      case 2: d = t_entry - ls.t_step; break;
+      default: d = (uint64_t)alg_bytes(op, ls.op_flags, a); break;
+    }
+    reinterpret_cast<uint64_t*>(a.prof)[threadIdx.x * 32 + op] += d;
+  }
 #pragma unroll
   for (int kk = 0; kk < 2; kk++) {
     const int k = 2 * w + kk;
     const bool mx = (mm >> k) & 1;
+    const double* v = kk ? v1 : v0;
     double acc = mx ? -HUGE_VAL : 0.0;
-    if (ran) {
-      bool first = true;
-      for (uint32_t b = lane; b < nblocks; b += 64) {
-        const double v = a.partials[(size_t)b * kRes + k];
-        acc = first ? v : combine(acc, v, mx);
-        first = false;
-      }
-      if (first) acc = mx ? -HUGE_VAL : 0.0;
+    if (ran && lane < nblocks) {
+      acc = v[0];
+#pragma unroll
+      for (int j = 1; j < kPer; j++)
+        if (lane + 64 * j < nblocks) acc = combine(acc, v[j], mx);
     }
     acc = wave_reduce(acc, mx);
     if (lane == 0) res[k] = acc;
   }
   __syncthreads();
-  const bool live = ls.pc != PC_DONE && !ls.error;
   if (threadIdx.x == 0 && live) {
-    const uint64_t now = wall_clock64();
-    const int32_t op = ls.op;
-    if (ran && op >= 0 && op < 32) {
-      Prof* pf = a.prof;
-      pf->op_count[op]++;
-      pf->op_ticks[op] += now - ls.t_last;
-      pf->step_ticks[op] += now - ls.t_step;
-      pf->op_bytes[op] += alg_bytes(op, ls.op_flags, a);
-    }
     Ctl c{ls, a, res};
     c.advance();
     ls.t_last = wall_clock64();
@@ -226,6 +243,7 @@ int pick_blocks(const acd_dev_matrix* orig, const acd_dev_matrix* perm) {
   int cap = 8 * acd_dev_cu_count();
   if (want > 64) want = (want + 7) / 8 * 8;  // whole XCD rounds
   if (cap < 64) cap = 64;
+  if (cap > (int)kMaxBlocks) cap = (int)kMaxBlocks;
   if (want > cap) want = cap;
   if (want < 1) want = 1;
   return want;
@@ -308,9 +326,21 @@ void init_state(State* s, const acd_dev_lp* lp, const acd_solve_request* q) {
   }
 }
 
+// ACD_SOLVER_GRAPHS=0 launches the same (step, control) pairs directly on
+// the stream instead of replaying captured graphs (for tools that do not
+// trace graph kernel nodes).
+bool use_graphs() {
+  static int on = -1;
+  if (on < 0) {
+    const char* e = getenv("ACD_SOLVER_GRAPHS");
+    on = !(e && e[0] == '0');
+  }
+  return on;
+}
+
 int run_to_completion(acd_dev_lp* lp, int key, State* hs,
                       acd_solve_result* res, acd_event_sink sink, void* ctx) {
-  if (build_graph(lp, key)) return -1;
+  if (use_graphs() && build_graph(lp, key)) return -1;
   Args a = make_args(lp, key);
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
@@ -327,7 +357,15 @@ int run_to_completion(acd_dev_lp* lp, int key, State* hs,
   int chunk = 1;
   while (!rc) {
     for (int c = 0; c < chunk && !rc; c++) {
-      if (hipGraphLaunch(lp->graph[key], S()) != hipSuccess) rc = -1;
+      if (use_graphs()) {
+        if (hipGraphLaunch(lp->graph[key], S()) != hipSuccess) rc = -1;
+      } else {
+        for (int i = 0; i < kPairs; i++) {
+          launch_step(key, lp->blocks, S(), a);
+          k_ctl<<<1, 256, 0, S()>>>(a, (uint32_t)lp->blocks);
+        }
+        if (hipGetLastError() != hipSuccess) rc = -1;
+      }
       launches += 2 * kPairs;
     }
     if (rc) {
@@ -375,7 +413,7 @@ int run_to_completion(acd_dev_lp* lp, int key, State* hs,
       g_prof_us[i] = khz > 0 ? 1e3 * (double)pf.op_ticks[i] / khz : 0.0;
       g_prof_step_us[i] =
           khz > 0 ? 1e3 * (double)pf.step_ticks[i] / khz : 0.0;
-      g_prof_bytes[i] = pf.op_bytes[i];
+      g_prof_bytes[i] = (double)pf.op_bytes[i];
     }
   }
   if (res) {

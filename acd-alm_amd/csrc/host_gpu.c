@@ -25,6 +25,11 @@ int acd_copy_to_host(void * dst, const void * src, size_t bytes)
         return acd_dev_d2h(dst, src, bytes);
 }
 
+int acd_copy_device(void * dst, const void * src, size_t bytes)
+{
+        return acd_dev_d2d(dst, src, bytes) || acd_dev_sync();
+}
+
 int acd_spmv_device(const sparse_matrix_t * a, int transpose, double * d_y,
                     const double * d_x)
 {

@@ -165,12 +165,13 @@ struct State {
 };
 static_assert(sizeof(State) % 8 == 0, "State is copied as 64-bit words");
 
-// Per-op counters of one solve, kept beside the state.
+// Per-op counters of one solve, kept beside the state (four arrays of 32,
+// updated as 64-bit words by consecutive lanes of the control kernel).
 struct Prof {
   uint64_t op_count[32];        // launches per op
-  uint64_t op_ticks[32];        // wall-clock ticks per op (step + control)
+  uint64_t op_ticks[32];        // ticks from the op's issue to its control
   uint64_t step_ticks[32];      // ticks from step kernel start to control
-  double op_bytes[32];          // algorithmic bytes moved per op
+  uint64_t op_bytes[32];        // algorithmic bytes moved per op
 };
 
 struct Args {

@@ -126,6 +126,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--outer", type=int, default=1000,
                     help="ALM outer-iteration cap (the reference uses 1000)")
+    ap.add_argument("--copy-check", type=int, default=0, metavar="BYTES",
+                    help="first run one device-to-device copy of BYTES "
+                         "(calibrates hardware byte counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -137,6 +140,13 @@ def main():
     lib = capi.load()
     L = lib.lib
     prec = {"mixed": 1, "fp64": 0}[args.precision]
+
+    if args.copy_check:
+        src = L.acd_device_alloc(args.copy_check)
+        dst = L.acd_device_alloc(args.copy_check)
+        L.acd_copy_device(dst, src, args.copy_check)
+        L.acd_device_free(src)
+        L.acd_device_free(dst)
 
     t0 = time.perf_counter()
     lp = instances.set_cover()

@@ -31,6 +31,9 @@ void * acd_device_alloc(size_t bytes);          /* zero-filled */
 void acd_device_free(void * p);
 int acd_copy_to_device(void * dst, const void * src, size_t bytes);
 int acd_copy_to_host(void * dst, const void * src, size_t bytes);
+/* Device-to-device copy on the library stream (a plain copy kernel of
+ * known size, used to calibrate hardware byte counters). */
+int acd_copy_device(void * dst, const void * src, size_t bytes);
 /* d_y <- op(A) d_x on device pointers, enqueued on the library stream. */
 int acd_spmv_device(const sparse_matrix_t *, int transpose,
                     double * d_y, const double * d_x);

@@ -186,6 +186,26 @@ void acd_dev_last_profile(uint64_t * counts, double * us, int n);
  * launch latency), and algorithmic bytes moved. */
 void acd_dev_last_profile_ex(double * step_us, double * bytes, int n);
 
+/* ---- randomised coordinate descent (dev_cd.hip, approx/cd.c) ---- */
+/* Works on the caller-order vectors of an acd_dev_lp (x, rhs, weight,
+ * linear, lower, upper) and the unpermuted matrix. */
+struct acd_dev_cd;
+struct acd_dev_cd * acd_dev_cd_create(const struct acd_dev_matrix *,
+                                      struct acd_dev_lp *);
+void acd_dev_cd_free(struct acd_dev_cd *);
+/* Column set-up from the current weights; copies the n sampling
+ * probabilities to h_prob. */
+int acd_dev_cd_setup(struct acd_dev_cd *, double * h_prob);
+int acd_dev_cd_clamp(struct acd_dev_cd *);
+/* g, violation and the objective value of the current x. */
+int acd_dev_cd_gradient(struct acd_dev_cd *, double * value);
+/* ||projected gradient||, ||g||, ||violation|| of the last gradient. */
+int acd_dev_cd_norms(struct acd_dev_cd *, double * pg, double * ng,
+                     double * nviol);
+/* The coordinate steps for the sampled columns, in order. */
+int acd_dev_cd_steps(struct acd_dev_cd *, const uint32_t * h_cols,
+                     size_t ncols, double scale);
+
 #ifdef __cplusplus
 }
 #endif

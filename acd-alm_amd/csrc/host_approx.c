@@ -262,3 +262,125 @@ int mpg_solve(double * x, size_t n, approx_t * a, size_t niter,
                 for (int i = 0; i < 5; i++) OUT_diagnosis[i] = diag[i];
         return 2;
 }
+
+/* ---- randomised coordinate descent (approx/cd.c) ---- */
+
+/* approx/cd.c:27-40: the column list shuffled with the process's random(),
+ * as the original does (it may read one past the end when random()
+ * returns RAND_MAX; the list keeps a zeroed slot there). */
+static void shuffle_columns(uint32_t * data, size_t nmemb)
+{
+        for (size_t i = 0; i < nmemb; i++) {
+                size_t j = (1.0 * random() / RAND_MAX) * (nmemb - i);
+                j += i;
+                uint32_t t = data[i];
+                data[i] = data[j];
+                data[j] = t;
+        }
+}
+
+static void cd_fail(struct acd_dev_cd * cd, uint32_t * cols, double * prob)
+{
+        acd_dev_cd_free(cd);
+        free(cols);
+        free(prob);
+        acd_fail("cd_solve");
+}
+
+/* approx/cd.c:284-330; 0 on success with the value in *out. */
+static int cd_major_step(struct acd_dev_cd * cd, approx_t * a,
+                         const double * prob, uint32_t * cols, double offset,
+                         FILE * log, double scale, double * out)
+{
+        size_t n = a->nvars;
+        double value, pg, nv;
+        if (acd_dev_cd_clamp(cd) || acd_dev_cd_gradient(cd, &value))
+                return -1;
+        if (log != NULL) {
+                if (acd_dev_cd_norms(cd, &pg, NULL, &nv)) return -1;
+                fprintf(log, "\tEntering: %12g %12g %12g\n", value + offset,
+                        pg, nv);
+        }
+        size_t cnt = 0, maxn = 11 * n;
+        double total = 0;
+        for (size_t i = 0; i < n; i++) total += prob[i];
+        for (size_t i = 0; i < n; i++) {
+                size_t count = ceil(10.0 * n * prob[i] / total);
+                ACD_REQUIRE(cnt + count <= maxn,
+                            "cd_solve: sampled column list overflow");
+                for (size_t j = 0; j < count; j++) cols[cnt++] = (uint32_t)i;
+        }
+        memset(cols + cnt, 0, (maxn + 1 - cnt) * sizeof(uint32_t));
+        shuffle_columns(cols, cnt);
+        if (acd_dev_cd_steps(cd, cols, cnt, scale) || acd_dev_cd_clamp(cd)
+            || acd_dev_cd_gradient(cd, &value))
+                return -1;
+        if (log != NULL) {
+                if (acd_dev_cd_norms(cd, &pg, NULL, &nv)) return -1;
+                /* the original prints this line to stdout */
+                printf("\t\tLeaving:  %12g %12g %12g\n", value + offset, pg,
+                       nv);
+        }
+        *out = value + offset;
+        return 0;
+}
+
+/* approx/cd.c:332-385 */
+int cd_solve(double * x, size_t n, approx_t * a, size_t niter,
+             double max_pg, double max_value, double min_delta,
+             FILE * log, size_t period, double * OUT_diagnosis,
+             double offset, thread_pool_t * pool)
+{
+        (void)min_delta;
+        (void)pool;
+        period /= 10;
+        ACD_REQUIRE(n == a->nvars, "cd_solve: n does not match the instance");
+        if (acd_approx_upload(a)) acd_fail("cd_solve");
+        struct acd_dev_cd * cd = acd_dev_cd_create(&a->matrix->dev, a->dev);
+        double * prob = malloc(sizeof(double) * (n + 1));
+        uint32_t * cols = calloc(11 * n + 1, sizeof(uint32_t));
+        if (cd == NULL || prob == NULL || cols == NULL
+            || acd_dev_cd_setup(cd, prob)
+            || acd_dev_h2d(acd_dev_lp_vec(a->dev, ACD_V_X), x,
+                           n * sizeof(double)))
+                cd_fail(cd, cols, prob);
+        int reason = 0;
+        size_t i;
+        for (i = 0; i < niter; i++) {
+                int print = (i == 0) || (period && ((i + 1) % period == 0))
+                        || (i + 1 == niter);
+                if (print && log != NULL) fprintf(log, "\t%6zu ", i + 1);
+                double z = 0, pg = 0;
+                if (cd_major_step(cd, a, prob, cols, offset,
+                                  print ? log : NULL, .5, &z)
+                    || acd_dev_cd_norms(cd, &pg, NULL, NULL))
+                        cd_fail(cd, cols, prob);
+                if (z < max_value) {
+                        reason = 1;
+                        break;
+                }
+                if (pg < max_pg) {
+                        reason = 2;
+                        break;
+                }
+        }
+        if (OUT_diagnosis != NULL) {
+                double value, pg, ng;
+                if (acd_dev_cd_gradient(cd, &value)
+                    || acd_dev_cd_norms(cd, &pg, &ng, NULL))
+                        cd_fail(cd, cols, prob);
+                OUT_diagnosis[0] = value + offset;
+                OUT_diagnosis[1] = ng;
+                OUT_diagnosis[2] = pg;
+                OUT_diagnosis[3] = 0;
+                OUT_diagnosis[4] = i + 1;
+        }
+        if (acd_dev_d2h(x, acd_dev_lp_vec(a->dev, ACD_V_X),
+                        n * sizeof(double)))
+                cd_fail(cd, cols, prob);
+        acd_dev_cd_free(cd);
+        free(cols);
+        free(prob);
+        if (log) fflush(log);
+        return reason;
+}

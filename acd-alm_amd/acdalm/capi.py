@@ -34,14 +34,20 @@ class AlmOptions(C.Structure):
     """struct acd_alm_options (include/acd_gpu.h)."""
     _fields_ = [("stop_violation", C.c_double), ("stop_pg", C.c_double),
                 ("precision_floor", C.c_double), ("min_delta", C.c_double),
-                ("precision", C.c_int)]
+                ("precision", C.c_int), ("stop_kkt", C.c_double)]
+
+
+class BatchInfo(C.Structure):
+    """struct acd_batch_info (include/acd_gpu.h)."""
+    _fields_ = [("device_ms", C.c_double), ("launches", C.c_uint64),
+                ("inner_iterations", C.c_uint64), ("alg_bytes", C.c_double)]
 
 
 class SolveStats(C.Structure):
     """struct acd_solve_stats (include/acd_gpu.h)."""
     _fields_ = [("inner_iterations", C.c_uint64),
                 ("outer_iterations", C.c_uint64), ("launches", C.c_uint64),
-                ("device_ms", C.c_double)]
+                ("device_ms", C.c_double), ("kkt", C.c_double)]
 
 
 def _d(a):
@@ -177,6 +183,12 @@ class CLib:
             ("alm_solve_ex", C.c_int, [vp, sz, dp, sz, dp, sz, vp, dp,
                                        C.POINTER(AlmOptions)]),
             ("acd_last_solve_stats", None, [C.POINTER(SolveStats)]),
+            ("alm_solve_batch", C.c_int, [sz, C.POINTER(vp), sz,
+                                          C.POINTER(dp), C.POINTER(dp),
+                                          C.POINTER(dp), C.POINTER(C.c_int),
+                                          C.POINTER(AlmOptions), dp,
+                                          C.POINTER(C.c_uint64),
+                                          C.POINTER(BatchInfo)]),
             ("acd_alm_reset", C.c_int, [vp]),
             ("acd_spmv_variant_time", C.c_int, [vp, C.c_int, vp, vp, C.c_int,
                                                 C.c_int, C.POINTER(C.c_double)]),
@@ -333,6 +345,33 @@ class CLib:
         rc = self.lib.alm_solve_ex(a, niter, _d(x), x.shape[0], _d(lam),
                                    lam.shape[0], cap.fp, _d(diag), C.byref(o))
         return rc, x, lam, diag, cap.close()
+
+    def alm_solve_batch(self, alms, niter, x0s, lam0s, **opts):
+        """alm_solve_batch over ALM objects; returns ([(ret, x, lam, diag)],
+        done_ms, BatchInfo with .inner = per-LP inner iterations)."""
+        n = len(alms)
+        o = AlmOptions()
+        self.lib.acd_alm_default_options(C.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        xs = [f64(v).copy() for v in x0s]
+        ls = [f64(v).copy() for v in lam0s]
+        ds = [np.full(3, np.nan) for _ in range(n)]
+        arr_a = (vp * n)(*alms)
+        arr_x = (dp * n)(*[_d(v) for v in xs])
+        arr_l = (dp * n)(*[_d(v) for v in ls])
+        arr_d = (dp * n)(*[_d(v) for v in ds])
+        rets = (C.c_int * n)()
+        done = np.zeros(n)
+        inner = np.zeros(n, dtype=np.uint64)
+        info = BatchInfo()
+        self.lib.alm_solve_batch(n, arr_a, niter, arr_x, arr_l, arr_d, rets,
+                                 C.byref(o), _d(done),
+                                 inner.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                 C.byref(info))
+        info.inner = inner
+        return ([(int(rets[i]), xs[i], ls[i], ds[i]) for i in range(n)],
+                done, info)
 
     def profile(self):
         """{pass name: (launches, microseconds)} of the last solve."""

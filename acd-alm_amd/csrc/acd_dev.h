@@ -131,6 +131,7 @@ struct acd_solve_request {
         /* ALM only */
         double stop_violation, stop_pg, precision_floor, inner_min_delta;
         int precision;            /* ACD_DEV_FP64 / _MIXED / _FP32 */
+        double stop_kkt;          /* > 0: KKT stopping target (ALM) */
 };
 
 struct acd_solve_result {
@@ -139,6 +140,7 @@ struct acd_solve_result {
         double diag[5];
         uint64_t inner_iterations, outer_iterations, launches;
         double device_ms;
+        double kkt;               /* last device KKT residual or NaN */
 };
 
 enum acd_err {
@@ -185,6 +187,22 @@ void acd_dev_last_profile(uint64_t * counts, double * us, int n);
  * the control kernel's start (the step kernel's duration plus the control
  * launch latency), and algorithmic bytes moved. */
 void acd_dev_last_profile_ex(double * step_us, double * bytes, int n);
+
+/* ---- batched solves (dev_solver.hip) ---- */
+/* Runs request q (ALM mode) on every LP, one workgroup per LP, in repeated
+ * launches of 32 inner iterations until all are done.  res[i] receives
+ * each LP's result; done_ms[i] (optional) the device time from the batch
+ * start to that LP's completion. */
+struct acd_batch_stats {
+        double device_ms;          /* HIP events around all launches */
+        uint64_t launches;
+        uint64_t inner_iterations; /* summed over the LPs */
+        double alg_bytes;          /* algorithmic bytes, summed */
+};
+int acd_dev_solve_batch(struct acd_dev_lp * const * lps, size_t count,
+                        const struct acd_solve_request * q,
+                        struct acd_solve_result * res, double * done_ms,
+                        struct acd_batch_stats * stats);
 
 /* ---- randomised coordinate descent (dev_cd.hip, approx/cd.c) ---- */
 /* Works on the caller-order vectors of an acd_dev_lp (x, rhs, weight,

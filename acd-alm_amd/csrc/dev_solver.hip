@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 #include "acd_dev.h"
 #include "solver_ops.hpp"
 
@@ -151,6 +152,96 @@ __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
       dst[i] = src[i];
   }
 }
+
+// ---------------------------------------------------------------------------
+// Batched solves: one workgroup per LP runs the same ops and control
+// continuation with the workgroup as the group (its 4 waves split the
+// slices) and __syncthreads as the barrier, with the LP's argument and
+// state records in LDS.  A launch advances every unfinished LP by up to
+// `budget` inner iterations (and at most kBatchOps ops), then saves its
+// state, so launches can be repeated, and LPs added between them, until
+// every LP is done.
+constexpr uint32_t kBatchOps = 1u << 16;
+
+struct BlockG {
+  uint32_t lane, gwave, nwaves;
+};
+
+#ifndef ACD_BATCH_MIN_BLOCKS
+#define ACD_BATCH_MIN_BLOCKS 1
+#endif
+template <class VI, class VC, class MT>
+__global__ void __launch_bounds__(64 * kWPB, ACD_BATCH_MIN_BLOCKS)
+k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
+        uint64_t budget, uint32_t* done_count, uint64_t* t_done,
+        uint64_t* bytes) {
+  __shared__ Args la;
+  __shared__ State ls;
+  __shared__ double red[kWPB][kRes];
+  __shared__ double res[kRes];
+  const uint32_t lp = slots[blockIdx.x];
+  const uint32_t tid = threadIdx.x;
+  {
+    const uint64_t* sa = reinterpret_cast<const uint64_t*>(args + lp);
+    uint64_t* da = reinterpret_cast<uint64_t*>(&la);
+    for (uint32_t i = tid; i < sizeof(Args) / 8; i += 64 * kWPB) da[i] = sa[i];
+    const uint64_t* ss = reinterpret_cast<const uint64_t*>(states + lp);
+    uint64_t* ds = reinterpret_cast<uint64_t*>(&ls);
+    for (uint32_t i = tid; i < sizeof(State) / 8; i += 64 * kWPB) ds[i] = ss[i];
+    if (tid < kRes) res[tid] = 0.0;
+  }
+  __syncthreads();
+  if (ls.pc == PC_DONE || ls.error) return;
+  const uint64_t stop_at = ls.inner_total + budget;
+  uint64_t moved = 0;
+  BlockG g;
+  g.lane = tid & 63;
+  g.gwave = tid >> 6;
+  g.nwaves = kWPB;
+  for (uint32_t guard = 0; guard < kBatchOps; guard++) {
+    if (ls.op != OP_NONE) {
+      Ops<BlockG, VI, VC, MT> ops(g, la, ls);
+      ops.run();
+      const uint32_t mm = ls.op_maxmask;
+#pragma unroll
+      for (int k = 0; k < kRes; k++) {
+        const double v = wave_reduce(ops.p[k], (mm >> k) & 1);
+        if (g.lane == 0) red[g.gwave][k] = v;
+      }
+      __syncthreads();
+      if (tid < kRes) {
+        const bool mx = (mm >> tid) & 1;
+        double acc = red[0][tid];
+        for (int w = 1; w < kWPB; w++) acc = combine(acc, red[w][tid], mx);
+        res[tid] = acc;
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      if (ls.op != OP_NONE) moved += (uint64_t)alg_bytes(ls.op, ls.op_flags, la);
+      Ctl c{ls, la, res};
+      c.advance();
+    }
+    __syncthreads();
+    if (ls.pc == PC_DONE || ls.error || ls.op == OP_NONE ||
+        ls.inner_total >= stop_at)
+      break;
+  }
+  {
+    const uint64_t* ss = reinterpret_cast<const uint64_t*>(&ls);
+    uint64_t* ds = reinterpret_cast<uint64_t*>(states + lp);
+    for (uint32_t i = tid; i < sizeof(State) / 8; i += 64 * kWPB) ds[i] = ss[i];
+  }
+  if (tid == 0) {
+    bytes[lp] += moved;
+    if (ls.pc == PC_DONE || ls.error) {
+      t_done[lp] = wall_clock64();
+      atomicAdd(done_count, 1u);
+    }
+  }
+}
+
+__global__ void k_stamp(uint64_t* t) { *t = wall_clock64(); }
 
 Sell view(const acd_sell& op) {
   Sell v;
@@ -314,6 +405,8 @@ void init_state(State* s, const acd_dev_lp* lp, const acd_solve_request* q) {
   s->precision_floor = q->precision_floor;
   s->inner_min_delta = q->inner_min_delta;
   s->mixed = q->precision == ACD_DEV_FP32;
+  s->stop_kkt = q->stop_kkt;
+  s->kkt = NAN;
   s->op = OP_NONE;
   switch (q->mode) {
     case ACD_MODE_APPROX: s->pc = PC_A_INIT; break;
@@ -422,6 +515,7 @@ int run_to_completion(acd_dev_lp* lp, int key, State* hs,
     res->device_ms = ms;
     res->inner_iterations = hs->inner_total;
     res->outer_iterations = hs->outer;
+    res->kkt = hs->kkt;
     if (hs->mode == ACD_MODE_ALM) {
       res->ret = hs->alm_ret;
       for (int i = 0; i < 3; i++) res->diag[i] = hs->alm_diag[i];
@@ -453,6 +547,34 @@ int precision_key(acd_dev_lp* lp, int precision) {
       if (acd_dev_sell_make_fp32(op)) return -1;
   if (precision == ACD_DEV_FP32) return bf ? K_F32_BF16 : K_F32;
   return bf ? K_MIX_BF16 : K_MIX_F32;
+}
+
+void launch_batch(int key, uint32_t nblocks, const Args* args, State* st,
+                  const uint32_t* slots, uint64_t budget, uint32_t* done,
+                  uint64_t* t_done, uint64_t* bytes) {
+  const dim3 grid(nblocks), block(64 * kWPB);
+  switch (key) {
+    case K_MIX_F32:
+      k_batch<double, float, float><<<grid, block, 0, S()>>>(
+          args, st, slots, budget, done, t_done, bytes);
+      break;
+    case K_MIX_BF16:
+      k_batch<double, float, uint16_t><<<grid, block, 0, S()>>>(
+          args, st, slots, budget, done, t_done, bytes);
+      break;
+    case K_F32:
+      k_batch<float, float, float><<<grid, block, 0, S()>>>(
+          args, st, slots, budget, done, t_done, bytes);
+      break;
+    case K_F32_BF16:
+      k_batch<float, float, uint16_t><<<grid, block, 0, S()>>>(
+          args, st, slots, budget, done, t_done, bytes);
+      break;
+    default:
+      k_batch<double, double, double><<<grid, block, 0, S()>>>(
+          args, st, slots, budget, done, t_done, bytes);
+      break;
+  }
 }
 
 }  // namespace
@@ -570,6 +692,194 @@ int acd_dev_solve(struct acd_dev_lp* lp, const struct acd_solve_request* q,
   init_state(&hs, lp, q);
   const int rc = run_to_completion(lp, key, &hs, res, sink, ctx);
   if (q->mode == ACD_MODE_ALM) lp->pkey = key;
+  return rc;
+}
+
+int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
+                        const struct acd_solve_request* q,
+                        struct acd_solve_result* res, double* done_ms,
+                        struct acd_batch_stats* stats) {
+  if (count == 0) return 0;
+  if (count > 0xffffffffu) return -1;
+  // One kernel instantiation per launch: every LP runs in the same key
+  // (BF16 values only when all of them are exact there).
+  int key = K_F64;
+  if (q->precision == ACD_DEV_MIXED || q->precision == ACD_DEV_FP32) {
+    bool all_bf = true;
+    for (size_t i = 0; i < count && all_bf; i++) {
+      const int k = precision_key(lps[i], q->precision);
+      if (k < 0) return -1;
+      all_bf = k == K_MIX_BF16 || k == K_F32_BF16;
+    }
+    if (!all_bf)
+      for (size_t i = 0; i < count; i++) {
+        acd_dev_matrix* pm = const_cast<acd_dev_matrix*>(lps[i]->perm);
+        acd_dev_matrix* om = const_cast<acd_dev_matrix*>(lps[i]->orig);
+        if (acd_dev_sell_make_fp32(&pm->a) || acd_dev_sell_make_fp32(&pm->at) ||
+            acd_dev_sell_make_fp32(&om->a) || acd_dev_sell_make_fp32(&om->at))
+          return -1;
+      }
+    key = q->precision == ACD_DEV_FP32 ? (all_bf ? K_F32_BF16 : K_F32)
+                                       : (all_bf ? K_MIX_BF16 : K_MIX_F32);
+  }
+  Args* h_args = (Args*)calloc(count, sizeof(Args));
+  State* h_st = (State*)calloc(count, sizeof(State));
+  uint32_t* h_slots = (uint32_t*)malloc(count * sizeof(uint32_t));
+  uint64_t* h_done = (uint64_t*)calloc(count, sizeof(uint64_t));
+  uint64_t* h_bytes = (uint64_t*)calloc(count, sizeof(uint64_t));
+  Args* d_args = nullptr;
+  State* d_st = nullptr;
+  uint32_t *d_slots = nullptr, *d_count = nullptr;
+  uint64_t *d_done = nullptr, *d_bytes = nullptr, *d_t0 = nullptr;
+  int rc = 0;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  if (!h_args || !h_st || !h_slots || !h_done || !h_bytes ||
+      hipMalloc((void**)&d_args, count * sizeof(Args)) != hipSuccess ||
+      hipMalloc((void**)&d_st, count * sizeof(State)) != hipSuccess ||
+      hipMalloc((void**)&d_slots, count * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void**)&d_count, sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void**)&d_done, count * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc((void**)&d_bytes, count * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc((void**)&d_t0, sizeof(uint64_t)) != hipSuccess) {
+    acd_dev_record_error(-1, "batch allocation");
+    rc = -1;
+  }
+  uint64_t launches = 0;
+  if (!rc) {
+    for (size_t i = 0; i < count; i++) {
+      h_args[i] = make_args(lps[i], key);
+      h_args[i].ev = nullptr;
+      h_args[i].ev_cap = 0;
+      h_args[i].st = d_st + i;
+      h_args[i].prof = nullptr;
+      h_args[i].partials = nullptr;
+      acd_solve_request qi = *q;
+      qi.log = 0;
+      init_state(&h_st[i], lps[i], &qi);
+      h_slots[i] = (uint32_t)i;
+    }
+    // Largest LPs first: workgroups are dispatched in launch order, so the
+    // long solves start at once instead of trailing the batch.
+    std::sort(h_slots, h_slots + count, [&](uint32_t x, uint32_t y) {
+      const uint64_t nx = lps[x]->orig->nnz, ny = lps[y]->orig->nnz;
+      return nx != ny ? nx > ny : x < y;
+    });
+    if (hipMemcpyAsync(d_args, h_args, count * sizeof(Args),
+                       hipMemcpyHostToDevice, S()) != hipSuccess ||
+        hipMemcpyAsync(d_st, h_st, count * sizeof(State),
+                       hipMemcpyHostToDevice, S()) != hipSuccess ||
+        hipMemcpyAsync(d_slots, h_slots, count * sizeof(uint32_t),
+                       hipMemcpyHostToDevice, S()) != hipSuccess ||
+        hipMemsetAsync(d_count, 0, sizeof(uint32_t), S()) != hipSuccess ||
+        hipMemsetAsync(d_bytes, 0, count * sizeof(uint64_t), S()) !=
+            hipSuccess ||
+        hipStreamSynchronize(S()) != hipSuccess)
+      rc = -1;
+  }
+  // Relaunch over the unfinished LPs (compacted between rounds) until all
+  // are done; the budget per launch is 32 inner iterations.
+  uint32_t active = (uint32_t)count;
+  uint32_t* live = h_slots;
+  uint32_t done = 0;
+  if (!rc) {
+    hipEventRecord(e0, S());
+    k_stamp<<<1, 1, 0, S()>>>(d_t0);
+    for (int round = 0; !rc && active; round++) {
+      for (int l = 0; l < 8; l++) {
+        launch_batch(key, active, d_args, d_st, d_slots, 32, d_count, d_done,
+                     d_bytes);
+        launches++;
+      }
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(&done, d_count, sizeof done, hipMemcpyDeviceToHost,
+                         S()) != hipSuccess ||
+          hipStreamSynchronize(S()) != hipSuccess) {
+        acd_dev_record_error(-1, "batch launch");
+        rc = -1;
+        break;
+      }
+      if (done >= count) break;
+      // Every 16 rounds, drop finished LPs from the launch list.
+      if (round % 16 == 15) {
+        if (hipMemcpy(h_st, d_st, count * sizeof(State),
+                      hipMemcpyDeviceToHost) != hipSuccess) {
+          rc = -1;
+          break;
+        }
+        uint32_t k = 0;
+        for (uint32_t i = 0; i < active; i++) {
+          const uint32_t lp = live[i];
+          if (h_st[lp].pc != PC_DONE && !h_st[lp].error) live[k++] = lp;
+        }
+        active = k;
+        if (active &&
+            hipMemcpyAsync(d_slots, live, active * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, S()) != hipSuccess)
+          rc = -1;
+      }
+    }
+    hipEventRecord(e1, S());
+    hipEventSynchronize(e1);
+  }
+  float ms = 0;
+  if (!rc) hipEventElapsedTime(&ms, e0, e1);
+  uint64_t t0 = 0;
+  if (!rc &&
+      (hipMemcpy(h_st, d_st, count * sizeof(State), hipMemcpyDeviceToHost) !=
+           hipSuccess ||
+       hipMemcpy(h_done, d_done, count * sizeof(uint64_t),
+                 hipMemcpyDeviceToHost) != hipSuccess ||
+       hipMemcpy(h_bytes, d_bytes, count * sizeof(uint64_t),
+                 hipMemcpyDeviceToHost) != hipSuccess ||
+       hipMemcpy(&t0, d_t0, sizeof t0, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = -1;
+  if (!rc) {
+    int khz = 100000;
+    hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate,
+                          acd_dev_device());
+    uint64_t inner = 0, total_bytes = 0;
+    for (size_t i = 0; i < count; i++) {
+      const State& hs = h_st[i];
+      acd_solve_result& r = res[i];
+      memset(&r, 0, sizeof r);
+      r.error = hs.error;
+      r.ret = hs.alm_ret;
+      for (int k = 0; k < 3; k++) r.diag[k] = hs.alm_diag[k];
+      r.inner_iterations = hs.inner_total;
+      r.outer_iterations = hs.outer;
+      r.kkt = hs.kkt;
+      r.launches = launches;
+      r.device_ms = ms;
+      if (done_ms)
+        done_ms[i] = h_done[i] >= t0 && khz > 0
+                         ? (double)(h_done[i] - t0) / khz
+                         : -1.0;
+      inner += hs.inner_total;
+      total_bytes += h_bytes[i];
+    }
+    if (stats) {
+      stats->device_ms = ms;
+      stats->launches = launches;
+      stats->inner_iterations = inner;
+      stats->alg_bytes = (double)total_bytes;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(d_args);
+  hipFree(d_st);
+  hipFree(d_slots);
+  hipFree(d_count);
+  hipFree(d_done);
+  hipFree(d_bytes);
+  hipFree(d_t0);
+  free(h_args);
+  free(h_st);
+  free(h_slots);
+  free(h_done);
+  free(h_bytes);
   return rc;
 }
 

@@ -84,6 +84,7 @@ void acd_alm_default_options(struct acd_alm_options * o)
         o->precision_floor = 1e-7;
         o->min_delta = 1e-11;
         o->precision = ACD_FP64;
+        o->stop_kkt = 0;
 }
 
 int alm_solve_ex(alm_t * alm, size_t niter, double * x, size_t nvars,
@@ -115,6 +116,7 @@ int alm_solve_ex(alm_t * alm, size_t niter, double * x, size_t nvars,
         q.precision_floor = o.precision_floor;
         q.inner_min_delta = o.min_delta;
         q.precision = o.precision;
+        q.stop_kkt = o.stop_kkt;
         struct acd_solve_result r;
         struct acd_log_ctx ctx = {log, 0};
         int rc = acd_dev_solve(dev, &q, &r, acd_log_sink, &ctx);
@@ -133,6 +135,86 @@ int alm_solve_ex(alm_t * alm, size_t niter, double * x, size_t nvars,
                 for (int i = 0; i < 3; i++) OUT_diagnosis[i] = r.diag[i];
         if (log) fflush(log);
         return r.ret;
+}
+
+static void upload_alm(alm_t * alm, const double * x, const double * lambda)
+{
+        approx_t * a = alm->approx;
+        struct acd_dev_lp * dev = a->dev;
+        size_t m = alm->nrhs, n = alm->nvars;
+        if (acd_approx_upload(a)
+            || acd_dev_h2d(acd_dev_lp_vec(dev, ACD_V_ALM_LINEAR), alm->linear, n * 8)
+            || acd_dev_h2d(acd_dev_lp_vec(dev, ACD_V_LAM_LO), alm->lambda_lower, m * 8)
+            || acd_dev_h2d(acd_dev_lp_vec(dev, ACD_V_LAM_HI), alm->lambda_upper, m * 8)
+            || acd_dev_h2d(acd_dev_lp_vec(dev, ACD_V_X), x, n * 8)
+            || acd_dev_h2d(acd_dev_lp_vec(dev, ACD_V_LAMBDA), lambda, m * 8))
+                acd_fail("alm_solve: upload");
+}
+
+static void download_alm(alm_t * alm, double * x, double * lambda,
+                         size_t niter)
+{
+        approx_t * a = alm->approx;
+        struct acd_dev_lp * dev = a->dev;
+        size_t m = alm->nrhs, n = alm->nvars;
+        if (acd_dev_d2h(x, acd_dev_lp_vec(dev, ACD_V_X), n * 8)
+            || acd_dev_d2h(lambda, acd_dev_lp_vec(dev, ACD_V_LAMBDA), m * 8)
+            || acd_dev_d2h(a->weight, acd_dev_lp_vec(dev, ACD_V_WEIGHT), m * 8)
+            || (niter && acd_dev_d2h(a->linear,
+                                     acd_dev_lp_vec(dev, ACD_V_LINEAR), n * 8)))
+                acd_fail("alm_solve: download");
+}
+
+int alm_solve_batch(size_t count, alm_t * const * alms, size_t niter,
+                    double * const * x, double * const * lambda,
+                    double * const * OUT_diagnosis, int * OUT_ret,
+                    const struct acd_alm_options * opt,
+                    double * OUT_done_ms, uint64_t * OUT_inner,
+                    struct acd_batch_info * OUT_info)
+{
+        struct acd_alm_options o;
+        if (opt) o = *opt;
+        else acd_alm_default_options(&o);
+        if (count == 0) return 0;
+        struct acd_dev_lp ** lps = calloc(count, sizeof(*lps));
+        struct acd_solve_result * res = calloc(count, sizeof(*res));
+        if (lps == NULL || res == NULL) acd_fail("alm_solve_batch: malloc");
+        for (size_t i = 0; i < count; i++) {
+                upload_alm(alms[i], x[i], lambda[i]);
+                lps[i] = alms[i]->approx->dev;
+        }
+        struct acd_solve_request q;
+        memset(&q, 0, sizeof q);
+        q.mode = ACD_MODE_ALM;
+        q.niter = niter;
+        q.stop_violation = o.stop_violation;
+        q.stop_pg = o.stop_pg;
+        q.precision_floor = o.precision_floor;
+        q.inner_min_delta = o.min_delta;
+        q.precision = o.precision;
+        q.stop_kkt = o.stop_kkt;
+        struct acd_batch_stats st;
+        memset(&st, 0, sizeof st);
+        int rc = acd_dev_solve_batch(lps, count, &q, res, OUT_done_ms, &st);
+        if (rc) acd_fail("alm_solve_batch: device solve failed");
+        for (size_t i = 0; i < count; i++) {
+                acd_check_result(&res[i], 0, "alm_solve_batch");
+                download_alm(alms[i], x[i], lambda[i], niter);
+                if (OUT_diagnosis != NULL && OUT_diagnosis[i] != NULL)
+                        for (int k = 0; k < 3; k++)
+                                OUT_diagnosis[i][k] = res[i].diag[k];
+                if (OUT_ret != NULL) OUT_ret[i] = res[i].ret;
+                if (OUT_inner != NULL) OUT_inner[i] = res[i].inner_iterations;
+        }
+        if (OUT_info != NULL) {
+                OUT_info->device_ms = st.device_ms;
+                OUT_info->launches = st.launches;
+                OUT_info->inner_iterations = st.inner_iterations;
+                OUT_info->alg_bytes = st.alg_bytes;
+        }
+        free(lps);
+        free(res);
+        return 0;
 }
 
 /* alm.c:289-322 */

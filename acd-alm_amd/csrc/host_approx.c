@@ -170,6 +170,7 @@ void acd_record_stats(const struct acd_solve_result * r)
         g_stats.outer_iterations = r->outer_iterations;
         g_stats.launches = r->launches;
         g_stats.device_ms = r->device_ms;
+        g_stats.kkt = r->kkt;
 }
 
 void acd_last_solve_stats(struct acd_solve_stats * out)

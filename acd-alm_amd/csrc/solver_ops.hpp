@@ -65,7 +65,8 @@ enum Op : int32_t {
   OP_STEP_SIZES,
   OP_ALM_VIOL,
   OP_ARGMAX,
-  OP_LAM_W
+  OP_LAM_W,
+  OP_KKT
 };
 
 enum OpFlag : uint32_t {
@@ -98,7 +99,7 @@ enum Pc : int32_t {
   PC_MF1, PC_MF2,
   // ALM
   PC_L0, PC_L1, PC_LB0, PC_LB1, PC_LB2, PC_LB3,
-  PC_LE0, PC_LE1, PC_LE2, PC_LE3, PC_LE4,
+  PC_LE0, PC_LE1, PC_LE2, PC_LE3, PC_LE4, PC_LE5,
   // approx_update
   PC_U0, PC_U1, PC_U2
 };
@@ -155,6 +156,8 @@ struct State {
   double precision, prev_norm, max_viol, alm_pg, cx, norm, w1_pre;
   double stop_violation, stop_pg, precision_floor, inner_min_delta;
   int32_t mixed;  // FP32 iterates: inner stall test at the FP32 floor
+  double stop_kkt;  // > 0: also stop when the relative KKT residual <= it
+  double kkt;       // last relative KKT residual (NaN until computed)
   double alm_diag[3];
   int32_t alm_ret, inner_reason, upd, bump;
   double scale, argmax;
@@ -190,10 +193,11 @@ struct Args {
   Prof* prof;
   double* partials;             // [blocks][kRes]
   acd_event* ev;
-  uint32_t ev_cap;
+  uint32_t ev_cap;              // 0: events off (batched solves)
   uint64_t nnz;                 // nonzeros of A
   uint32_t si, sc, sv;          // bytes per iterate / constant / value
 };
+static_assert(sizeof(Args) % 8 == 0, "Args is copied as 64-bit words");
 
 // SSE2 maxpd(a, b) / minpd(a, b) of the reference's projections
 // (approx/step_project.inc:18-19, :53-54, approx/approx.c:272-273).
@@ -246,38 +250,74 @@ struct Ops {
   template <int NRHS, class MT2, class X, class E>
   __device__ __forceinline__ void spmv_rows_t(const Sell& M, const X* x0,
                                               const X* x1, E&& epi) {
-    for (uint32_t sl = g.gwave; sl < M.nslices; sl += g.nwaves) {
-      const uint64_t base = M.off[sl];
-      const uint32_t len = (uint32_t)((M.off[sl + 1] - base) >> 6);
+    uint32_t sl = g.gwave;
+    if (sl >= M.nslices) return;
+    uint64_t base = M.off[sl], end = M.off[sl + 1];
+    for (;;) {
+      // The next slice's bounds and each next chunk of 4 columns/values
+      // are loaded while the current chunk's gathers are in flight; the
+      // accumulation order is unchanged.
+      const uint32_t nsl = sl + g.nwaves;
+      uint64_t nbase = 0, nend = 0;
+      if (nsl < M.nslices) {
+        nbase = M.off[nsl];
+        nend = M.off[nsl + 1];
+      }
+      const uint32_t len = (uint32_t)((end - base) >> 6);
       uint64_t e = base + g.lane;
-      const uint32_t* c = M.col + e;
       double a0 = 0.0, a1 = 0.0;
       uint32_t k = 0;
-      for (; k + 4 <= len; k += 4, e += 256, c += 256) {
-        const double v0 = mval<MT2>(M, e), v1 = mval<MT2>(M, e + 64),
-                     v2 = mval<MT2>(M, e + 128), v3 = mval<MT2>(M, e + 192);
-        const uint32_t c0 = c[0], c1 = c[64], c2 = c[128], c3 = c[192];
-        const double p0 = x0[c0], p1 = x0[c1], p2 = x0[c2], p3 = x0[c3];
-        a0 = a0 + v0 * p0;
-        a0 = a0 + v1 * p1;
-        a0 = a0 + v2 * p2;
-        a0 = a0 + v3 * p3;
-        if (NRHS == 2) {
-          const double q0 = x1[c0], q1 = x1[c1], q2 = x1[c2], q3 = x1[c3];
-          a1 = a1 + v0 * q0;
-          a1 = a1 + v1 * q1;
-          a1 = a1 + v2 * q2;
-          a1 = a1 + v3 * q3;
+      if (len >= 4) {
+        uint32_t c0 = M.col[e], c1 = M.col[e + 64], c2 = M.col[e + 128],
+                 c3 = M.col[e + 192];
+        double v0 = mval<MT2>(M, e), v1 = mval<MT2>(M, e + 64),
+               v2 = mval<MT2>(M, e + 128), v3 = mval<MT2>(M, e + 192);
+        for (;;) {
+          const bool more = k + 8 <= len;
+          uint32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0;
+          double w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+          if (more) {
+            const uint64_t en = e + 256;
+            d0 = M.col[en];
+            d1 = M.col[en + 64];
+            d2 = M.col[en + 128];
+            d3 = M.col[en + 192];
+            w0 = mval<MT2>(M, en);
+            w1 = mval<MT2>(M, en + 64);
+            w2 = mval<MT2>(M, en + 128);
+            w3 = mval<MT2>(M, en + 192);
+          }
+          const double p0 = x0[c0], p1 = x0[c1], p2 = x0[c2], p3 = x0[c3];
+          a0 = a0 + v0 * p0;
+          a0 = a0 + v1 * p1;
+          a0 = a0 + v2 * p2;
+          a0 = a0 + v3 * p3;
+          if (NRHS == 2) {
+            const double q0 = x1[c0], q1 = x1[c1], q2 = x1[c2], q3 = x1[c3];
+            a1 = a1 + v0 * q0;
+            a1 = a1 + v1 * q1;
+            a1 = a1 + v2 * q2;
+            a1 = a1 + v3 * q3;
+          }
+          k += 4;
+          e += 256;
+          if (!more) break;
+          c0 = d0; c1 = d1; c2 = d2; c3 = d3;
+          v0 = w0; v1 = w1; v2 = w2; v3 = w3;
         }
       }
-      for (; k < len; k++, e += 64, c += 64) {
+      for (; k < len; k++, e += 64) {
         const double vv = mval<MT2>(M, e);
-        const uint32_t cc = *c;
+        const uint32_t cc = M.col[e];
         a0 = a0 + vv * (double)x0[cc];
         if (NRHS == 2) a1 = a1 + vv * (double)x1[cc];
       }
       const uint32_t r = sl * 64 + g.lane;
       if (r < M.nrows) epi(r, a0, a1);
+      if (nsl >= M.nslices) break;
+      sl = nsl;
+      base = nbase;
+      end = nend;
     }
   }
   // Working-copy operators (A, A') in the solver precision.  The ALM
@@ -678,6 +718,26 @@ struct Ops {
         });
         break;
       }
+      case OP_KKT: {  // relative KKT residual of (x, lambda); d = c - A'lambda
+        spmv_rows_t<1, MT, double>(a.AT0, a.lam, nullptr, [&](uint32_t j, double atl, double) {
+          const double c = a.alm_c[j];
+          const double d = c - atl;
+          const double x = a.x_io[j], lo = a.lower[j], up = a.upper[j];
+          const double pr = fmin(fmax(x - d, lo), up);
+          const double dp = fmax(d, 0.0), dn = fmax(-d, 0.0);
+          const bool lo_inf = !isfinite(lo), up_inf = !isfinite(up);
+          const double inf = (lo_inf ? dp : 0.0) + (up_inf ? dn : 0.0);
+          p[0] += (x - pr) * (x - pr);
+          p[1] += inf * inf;
+          p[2] += (lo_inf ? 0.0 : lo * dp) - (up_inf ? 0.0 : up * dn);
+          p[3] += c * c;
+        });
+        each(a.m, [&](uint32_t i) {
+          p[4] += a.rhs[i] * a.lam[i];
+          p[5] += a.rhs[i] * a.rhs[i];
+        });
+        break;
+      }
       default:
         break;
     }
@@ -739,6 +799,7 @@ __device__ inline double alg_bytes(int32_t op, uint32_t fl, const Args& a) {
     case OP_ALM_VIOL: return spA + 8 * n + 24 * m + 16 * n;
     case OP_ARGMAX: return 8 * m;
     case OP_LAM_W: return 48 * m + ((fl & F_UPD) ? 8 * m : 0);
+    case OP_KKT: return spT + 8 * m + 32 * n + 16 * m;
     default: return 0;
   }
 }
@@ -841,7 +902,7 @@ struct Ctl {
           s.pc = PC_F0;
           break;
         }
-        if (s.events + 8 > a.ev_cap) {  // let the host drain the log
+        if (a.ev_cap && s.events + 8 > a.ev_cap) {  // let the host drain the log
           s.events_full = 1;
           s.op = OP_NONE;
           s.pc = PC_I0;
@@ -1096,7 +1157,7 @@ struct Ctl {
           issue(OP_PERMUTE_OUT, F_CX, PC_MF1);
           break;
         }
-        if (s.events + 8 > a.ev_cap) {
+        if (a.ev_cap && s.events + 8 > a.ev_cap) {
           s.events_full = 1;
           s.op = OP_NONE;
           return;
@@ -1245,6 +1306,25 @@ struct Ctl {
         s.alm_diag[1] = s.max_viol;
         s.alm_diag[2] = s.alm_pg;
         if (s.max_viol < s.stop_violation && (stalled || s.alm_pg < s.stop_pg)) {
+          s.alm_ret = 0;
+          s.pc = PC_DONE;
+        } else if (s.stop_kkt > 0) {
+          issue(OP_KKT, 0, PC_LE5);
+        } else if (s.outer >= s.outer_niter) {
+          s.alm_ret = 1;
+          s.pc = PC_DONE;
+        } else {
+          s.pc = PC_LB0;
+        }
+        break;
+      }
+      case PC_LE5: {  // optional stop on the relative KKT residual
+        const double rp = s.norm / (1 + sqrt(res[5]));
+        const double rd = (sqrt(res[0]) + sqrt(res[1])) / (1 + sqrt(res[3]));
+        const double dual = res[4] + res[2];
+        const double rg = fabs(s.cx - dual) / (1 + fabs(s.cx) + fabs(dual));
+        s.kkt = fmax(rp, fmax(rd, rg));
+        if (s.kkt <= s.stop_kkt) {
           s.alm_ret = 0;
           s.pc = PC_DONE;
         } else if (s.outer >= s.outer_niter) {

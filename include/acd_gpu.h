@@ -74,12 +74,39 @@ struct acd_alm_options {
         double precision_floor;  /* 1e-7  (alm.c:244) */
         double min_delta;        /* 1e-11 (alm.c:256) */
         int precision;           /* enum acd_precision */
+        double stop_kkt;         /* 0 (off).  > 0: also stop once the relative
+                                    KKT residual max(r_p, r_d, r_gap) of
+                                    (x, lambda), evaluated on the device after
+                                    each outer iteration, is <= stop_kkt */
 };
 void acd_alm_default_options(struct acd_alm_options *);
 /* alm_solve with explicit stopping constants and precision. */
 int alm_solve_ex(alm_t *, size_t niter, double * x, size_t nvars,
                  double * lambda, size_t nconstraints, FILE * log,
                  double * OUT_diagnosis, const struct acd_alm_options *);
+
+/* ---- batched solves ---- */
+struct acd_batch_info {
+        double device_ms;            /* device time of the whole batch */
+        uint64_t launches;           /* batch kernel launches */
+        uint64_t inner_iterations;   /* summed over the LPs */
+        double alg_bytes;            /* algorithmic bytes moved, summed */
+};
+/* alm_solve_ex on `count` independent ALM objects at once: one workgroup
+ * per LP, every LP on the device together, repeated launches of 32 inner
+ * iterations until all are done (no extension of the reference; a loop
+ * of alm_solve_ex calls gives the same results up to reduction order).
+ * x[i], lambda[i] (in/out) and OUT_diagnosis[i] (double[3], may be NULL)
+ * are per LP; OUT_ret[i] receives alm_solve's return code.  OUT_done_ms
+ * and OUT_inner (optional, count entries) receive each LP's completion
+ * time from the batch start and its inner iteration count.  Each LP's result does not depend on the other LPs in the
+ * batch. */
+int alm_solve_batch(size_t count, alm_t * const * alms, size_t niter,
+                    double * const * x, double * const * lambda,
+                    double * const * OUT_diagnosis, int * OUT_ret,
+                    const struct acd_alm_options * options,
+                    double * OUT_done_ms, uint64_t * OUT_inner,
+                    struct acd_batch_info * OUT_info);
 
 /* Restore the row weights (1) and penalised linear term (0) that alm_make
  * starts from, so the next alm_solve repeats a fresh solve; the original
@@ -92,6 +119,8 @@ struct acd_solve_stats {
         uint64_t outer_iterations;   /* ALM iterations */
         uint64_t launches;           /* solver kernel launches */
         double device_ms;            /* time inside solver launches */
+        double kkt;                  /* last device KKT residual (stop_kkt
+                                        > 0), else NaN */
 };
 void acd_last_solve_stats(struct acd_solve_stats *);
 /* Time split of the last solve by device pass: for pass p < n, the number

@@ -1,0 +1,107 @@
+"""Batched solves (alm_solve_batch): many independent LPs, one workgroup
+each, in repeated launches of 32 inner iterations.
+
+Each LP's arithmetic depends only on that LP (its own workgroup, its own
+fixed-order reductions), so a result is bit-identical alone and in any
+mix; against the single-LP solver (grid-wide reductions in another
+order) and the reference's golden runs it agrees to the stopping rule.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from acdalm import instances as I
+from acdalm.kkt import kkt
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def make(gpu, lps):
+    mats = [gpu.make_matrix(lp, 0) for lp in lps]
+    alms = [gpu.alm_make(m, lp) for m, lp in zip(mats, lps)]
+    return mats, alms
+
+
+def free(gpu, mats, alms):
+    for a in alms:
+        gpu.alm_free(a)
+    for m in mats:
+        gpu.free_matrix(m)
+
+
+def batch(gpu, lps, precision, niter=1000):
+    mats, alms = make(gpu, lps)
+    out, done, info = gpu.alm_solve_batch(
+        alms, niter, [np.zeros(lp.ncols) for lp in lps],
+        [np.zeros(lp.nrows) for lp in lps], precision=precision)
+    free(gpu, mats, alms)
+    return out, done, info
+
+
+def same(r1, r2):
+    return (r1[0] == r2[0] and np.array_equal(r1[1], r2[1])
+            and np.array_equal(r1[2], r2[2]) and np.array_equal(r1[3], r2[3]))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_result_is_the_same_alone_and_in_the_mix(gpu, precision):
+    lps = I.batch(12, seed=5, mmin=20, mmax=150)
+    print(f"[batch] mix precision={precision}", flush=True)
+    mixed, done, info = batch(gpu, lps, precision)
+    assert info.launches > 0 and np.all(done > 0)
+    for i in (0, 5, 11):
+        alone, _, _ = batch(gpu, [lps[i]], precision)
+        assert same(mixed[i], alone[0]), i
+    # another mix: reversed order with two extra LPs in front
+    extra = I.batch(2, seed=6, mmin=20, mmax=150)
+    other, _, _ = batch(gpu, extra + lps[::-1], precision)
+    for i in range(len(lps)):
+        assert same(mixed[i], other[2 + len(lps) - 1 - i]), i
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_agrees_with_single_solver(gpu, precision):
+    lps = I.batch(8, seed=7, mmin=30, mmax=200)
+    print(f"[batch] vs single precision={precision}", flush=True)
+    res, _, _ = batch(gpu, lps, precision)
+    for lp, (ret, x, lam, diag) in zip(lps, res):
+        m = gpu.make_matrix(lp, 0)
+        a = gpu.alm_make(m, lp)
+        rs, xs, ls, ds, _ = gpu.alm_solve_ex(a, 1000, np.zeros(lp.ncols),
+                                             np.zeros(lp.nrows),
+                                             precision=precision)
+        gpu.alm_free(a)
+        gpu.free_matrix(m)
+        assert ret == rs == 0
+        assert abs(diag[0] - ds[0]) <= 1e-6 * max(1.0, abs(ds[0]))
+        k, ks = kkt(lp, x, lam)["kkt"], kkt(lp, xs, ls)["kkt"]
+        assert k <= max(1e-6, 10 * ks), (k, ks)
+
+
+def test_batch_fp64_matches_reference_objectives(gpu):
+    cases = {"small": I.random_lp(60, 150, 4, 21),
+             "setcover": I.set_cover(80, 300, 6),
+             "mcf": I.multicommodity(4, 2, 7)}
+    names = sorted(cases)
+    print("[batch] reference objectives", flush=True)
+    res, _, _ = batch(gpu, [cases[n] for n in names], 0)
+    for n, (ret, x, lam, diag) in zip(names, res):
+        ref = np.load(os.path.join(GOLD, f"alm_{n}.npz"), allow_pickle=False)
+        assert ret == int(ref["rc"]) == 0
+        assert abs(diag[0] - ref["diag"][0]) <= 1e-6 * max(1.0, abs(ref["diag"][0]))
+        assert diag[1] < 1e-5
+
+
+def test_batch_kkt_stopping_rule(gpu):
+    lps = I.batch(6, seed=8, mmin=30, mmax=200)
+    print("[batch] stop_kkt", flush=True)
+    mats, alms = make(gpu, lps)
+    out, done, info = gpu.alm_solve_batch(
+        alms, 1000, [np.zeros(lp.ncols) for lp in lps],
+        [np.zeros(lp.nrows) for lp in lps], precision=0, stop_kkt=1e-6)
+    free(gpu, mats, alms)
+    for lp, (ret, x, lam, diag) in zip(lps, out):
+        assert ret == 0
+        assert kkt(lp, x, lam)["kkt"] <= 1e-6

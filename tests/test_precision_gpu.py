@@ -91,3 +91,23 @@ def test_fp32_iterates_error_is_bounded(gpu, name):
     assert np.isfinite(diag[0])
     assert rel <= 1e-4
     assert k <= 1e-2
+
+
+@pytest.mark.parametrize("name", ["small", "c0", "mcf"])
+def test_kkt_stopping_rule(gpu, name):
+    """stop_kkt: the device evaluates the relative KKT residual after every
+    outer iteration (the same formula as acdalm.kkt) and stops at the
+    target; the host evaluation of the returned (x, lambda) agrees."""
+    lp = CASES[name]
+    print(f"[precision] stop_kkt {name}", flush=True)
+    m = gpu.make_matrix(lp, 0)
+    alm = gpu.alm_make(m, lp)
+    rc, x, lam, diag, _ = gpu.alm_solve_ex(
+        alm, 1000, np.zeros(lp.ncols), np.zeros(lp.nrows), stop_kkt=1e-6)
+    dev = gpu.stats().kkt
+    gpu.alm_free(alm)
+    gpu.free_matrix(m)
+    host = kkt(lp, x, lam)["kkt"]
+    assert rc == 0
+    assert np.isfinite(dev) and dev <= 1e-6
+    assert abs(dev - host) <= 1e-6 * host + 1e-13, (dev, host)

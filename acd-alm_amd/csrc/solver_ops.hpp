@@ -1305,11 +1305,12 @@ struct Ctl {
         s.alm_diag[0] = s.cx;
         s.alm_diag[1] = s.max_viol;
         s.alm_diag[2] = s.alm_pg;
-        if (s.max_viol < s.stop_violation && (stalled || s.alm_pg < s.stop_pg)) {
+        if (s.stop_kkt > 0) {  // the KKT target replaces the absolute rule
+          issue(OP_KKT, 0, PC_LE5);
+        } else if (s.max_viol < s.stop_violation &&
+                   (stalled || s.alm_pg < s.stop_pg)) {
           s.alm_ret = 0;
           s.pc = PC_DONE;
-        } else if (s.stop_kkt > 0) {
-          issue(OP_KKT, 0, PC_LE5);
         } else if (s.outer >= s.outer_niter) {
           s.alm_ret = 1;
           s.pc = PC_DONE;
@@ -1318,7 +1319,7 @@ struct Ctl {
         }
         break;
       }
-      case PC_LE5: {  // optional stop on the relative KKT residual
+      case PC_LE5: {  // stop on the relative KKT residual (stop_kkt > 0)
         const double rp = s.norm / (1 + sqrt(res[5]));
         const double rd = (sqrt(res[0]) + sqrt(res[1])) / (1 + sqrt(res[3]));
         const double dual = res[4] + res[2];

@@ -6,9 +6,9 @@ A step is one complete ``alm_solve`` (ALM outer loop, APPROX inner loop,
 SpMV) of the set-cover LP relaxation (m = 200,000 rows, n = 1,000,000
 columns plus m slack columns, 0/1 coefficients, 3-8 nonzeros per column,
 c ~ U[1,100], 0 <= x <= 1; seeded synthetic instance), from x = 0 and
-lambda = 0 with the reference's stopping rule (alm.c:303-319), or
-earlier once the relative KKT residual of (x, lambda), evaluated on the
-device after each outer iteration, is at most 1e-6 (--stop-kkt).  The matrix,
+lambda = 0 until the relative KKT residual of (x, lambda), evaluated on
+the device after each outer iteration, is at most 1e-6 (--stop-kkt; 0
+selects the reference's absolute rule, alm.c:303-319, instead).  The matrix,
 the problem vectors and all solver state are resident in HBM before the
 timed region; each step restores the multiplier weights and penalised cost
 (acd_alm_reset) and solves again.
@@ -117,6 +117,71 @@ def cpu_baseline(lp, inner_iterations, iters, threads):
                           inner_iterations))}
 
 
+def batch_leg(lib, count, prec, stop_kkt, steps=1, fp64_sample=8):
+    """Batched solve of `count` independent LPs (m log-uniform in [50, 2000],
+    n = 2-3 m, 5 nonzeros per column, random +-U[0.5, 2] values; seeded),
+    all device-resident, to the relative KKT target.  Also times the plain
+    FP64 single-LP form (alm_solve_ex, one LP at a time) on the first
+    `fp64_sample` LPs for the ratio."""
+    from acdalm import instances, kkt
+    L = lib.lib
+    t0 = time.perf_counter()
+    lps = instances.batch(count, seed=4)
+    mats = [lib.make_matrix(lp, 0) for lp in lps]
+    alms = [lib.alm_make(m, lp) for m, lp in zip(mats, lps)]
+    setup = time.perf_counter() - t0
+    x0s = [np.zeros(lp.ncols) for lp in lps]
+    l0s = [np.zeros(lp.nrows) for lp in lps]
+    dev_ms, done_all, kkts, inner, alg = 0.0, [], [], 0, 0.0
+    for _ in range(steps):
+        for a in alms:
+            L.acd_alm_reset(a)
+        # With FP32-rounded matrix values the device measures the KKT
+        # residual of the rounded LP; a 10% margin keeps the true LP's
+        # residual (checked below on the host) under the target.
+        out, done, info = lib.alm_solve_batch(
+            alms, 1000, x0s, l0s, precision=prec,
+            stop_kkt=stop_kkt * (0.9 if prec else 1.0))
+        dev_ms += info.device_ms
+        done_all.append(done)
+        inner += int(info.inner_iterations)
+        alg += info.alg_bytes
+        bad = [i for i, r in enumerate(out) if r[0] != 0]
+        if bad:
+            raise RuntimeError("batch: %d LPs did not converge" % len(bad))
+    kkts = [kkt.kkt(lp, r[1], r[2])["kkt"] for lp, r in zip(lps, out)]
+    done = np.concatenate(done_all)
+    # the plain FP64 form on a sample, same target
+    t = time.perf_counter()
+    for a, lp in zip(alms[:fp64_sample], lps[:fp64_sample]):
+        L.acd_alm_reset(a)
+        lib.alm_solve_ex(a, 1000, np.zeros(lp.ncols), np.zeros(lp.nrows),
+                         precision=0, stop_kkt=stop_kkt)
+    single = fp64_sample / (time.perf_counter() - t)
+    for a in alms:
+        lib.alm_free(a)
+    for m in mats:
+        lib.free_matrix(m)
+    rate = count * steps / (dev_ms * 1e-3)
+    gbs = alg / (dev_ms * 1e-3) / 1e9
+    return {
+        "workload": "%d independent LPs, m log-uniform [50,2000], n=2-3m, "
+                    "5 nnz/column" % count,
+        "precision": "mixed" if prec == 1 else "fp64",
+        "solves_per_s": rate, "device_ms_per_batch": dev_ms / steps,
+        "p50_ms": float(np.percentile(done, 50)),
+        "p99_ms": float(np.percentile(done, 99)),
+        "kkt_max": float(max(kkts)), "inner_iterations": inner // steps,
+        "roofline": {"kernel": "k_batch", "achieved": round(gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)},
+        "fp64_single_lp_solves_per_s": single,
+        "fp64_single_lp_sample": fp64_sample,
+        "vs_fp64_single_lp": rate / single,
+        "setup_s": setup,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,10 +192,15 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stop-kkt", type=float, default=1e-6,
-                    help="also stop once the relative KKT residual is at "
-                         "most this (0: the reference's rule only)")
+                    help="stop once the relative KKT residual is at most "
+                         "this (0: the reference's absolute rule)")
     ap.add_argument("--outer", type=int, default=1000,
                     help="ALM outer-iteration cap (the reference uses 1000)")
+    ap.add_argument("--workload", choices=["setcover", "batch"],
+                    default="setcover")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="setcover workload: skip the batched-solve leg")
     ap.add_argument("--copy-check", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES "
                          "(calibrates hardware byte counters)")
@@ -145,6 +215,26 @@ def main():
     lib = capi.load()
     L = lib.lib
     prec = {"mixed": 1, "fp64": 0}[args.precision]
+
+    if args.workload == "batch":
+        L.acd_synchronize()
+        t = time.perf_counter()
+        b = batch_leg(lib, args.batch_count, prec, args.stop_kkt,
+                      steps=args.steps)
+        total = time.perf_counter() - t
+        print(json.dumps({
+            "metric": METRIC, "value": b["solves_per_s"], "unit": "solves/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": b["device_ms_per_batch"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic (seeded generator)",
+            "config": {"workload": b["workload"], "precision": args.precision,
+                       "stop": "relative KKT <= %g" % args.stop_kkt,
+                       "parallelism": "single GPU"},
+            "p50_ms": b["p50_ms"], "p99_ms": b["p99_ms"],
+            "roofline": dict(b["roofline"], bound="hbm", traffic=None),
+            "cpu_baseline": None, "batch": b, "wall_s": total}), flush=True)
+        return
 
     if args.copy_check:
         src = L.acd_device_alloc(args.copy_check)
@@ -224,6 +314,12 @@ def main():
             "share": round(step_us[i] / step_us.sum(), 3)}
             for i in top if step_us[i] > 0},
     }
+    batch = None
+    if not args.no_batch:
+        try:
+            batch = batch_leg(lib, args.batch_count, 1, args.stop_kkt or 1e-6)
+        except Exception as e:  # reported, never silently replaced
+            batch = {"error": str(e)[-300:]}
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -244,7 +340,7 @@ def main():
         "config": {"workload": "set-cover LP relaxation m=200000 n=1000000"
                                "+200000 slack, 0/1 A, 3-8 nnz/column",
                    "precision": args.precision,
-                   "stop": "reference rule or relative KKT <= %g" % args.stop_kkt
+                   "stop": "relative KKT <= %g" % args.stop_kkt
                            if args.stop_kkt > 0 else "reference rule",
                    "nnz": int(lp.nnz), "m": int(lp.nrows), "n": int(lp.ncols),
                    "parallelism": "single GPU"},
@@ -256,6 +352,7 @@ def main():
         "solver_device_ms": device_ms / args.steps,
         "roofline": roofline,
         "spmv": spmv,
+        "batch": batch,
         "cpu_baseline": cpu,
         "setup_s": setup_s,
     }

@@ -74,8 +74,9 @@ struct acd_alm_options {
         double precision_floor;  /* 1e-7  (alm.c:244) */
         double min_delta;        /* 1e-11 (alm.c:256) */
         int precision;           /* enum acd_precision */
-        double stop_kkt;         /* 0 (off).  > 0: also stop once the relative
-                                    KKT residual max(r_p, r_d, r_gap) of
+        double stop_kkt;         /* 0: the reference's rule above.  > 0:
+                                    instead stop once the relative KKT
+                                    residual max(r_p, r_d, r_gap) of
                                     (x, lambda), evaluated on the device after
                                     each outer iteration, is <= stop_kkt */
 };

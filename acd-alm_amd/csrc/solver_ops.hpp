@@ -247,9 +247,21 @@ struct Ops {
   // +0.0 with separate multiply and add (the reference's per-row order,
   // spmv/spmv_block.c:160-254); epi(row, acc0, acc1).  MT2/X are the value
   // and gathered-vector types (the ALM passes use double for both).
+  struct NoPre {};
   template <int NRHS, class MT2, class X, class E>
   __device__ __forceinline__ void spmv_rows_t(const Sell& M, const X* x0,
                                               const X* x1, E&& epi) {
+    spmv_rows_pt<NRHS, MT2, X>(
+        M, x0, x1, [](uint32_t) { return NoPre{}; },
+        [&](uint32_t r, double a0, double a1, NoPre) { epi(r, a0, a1); });
+  }
+  // Same, with pre(row) called at the start of each slice so the row's
+  // epilogue operands are loaded while its products are gathered; epi
+  // receives what pre returned.
+  template <int NRHS, class MT2, class X, class PreF, class E>
+  __device__ __forceinline__ void spmv_rows_pt(const Sell& M, const X* x0,
+                                               const X* x1, PreF&& pre,
+                                               E&& epi) {
     uint32_t sl = g.gwave;
     if (sl >= M.nslices) return;
     uint64_t base = M.off[sl], end = M.off[sl + 1];
@@ -264,6 +276,9 @@ struct Ops {
         nend = M.off[nsl + 1];
       }
       const uint32_t len = (uint32_t)((end - base) >> 6);
+      const uint32_t r = sl * 64 + g.lane;
+      const bool live = r < M.nrows;
+      const auto pv = pre(live ? r : 0u);
       uint64_t e = base + g.lane;
       double a0 = 0.0, a1 = 0.0;
       uint32_t k = 0;
@@ -312,13 +327,18 @@ struct Ops {
         a0 = a0 + vv * (double)x0[cc];
         if (NRHS == 2) a1 = a1 + vv * (double)x1[cc];
       }
-      const uint32_t r = sl * 64 + g.lane;
-      if (r < M.nrows) epi(r, a0, a1);
+      if (live) epi(r, a0, a1, pv);
       if (nsl >= M.nslices) break;
       sl = nsl;
       base = nbase;
       end = nend;
     }
+  }
+  template <int NRHS, class PreF, class E>
+  __device__ __forceinline__ void spmv_rows_pre(const Sell& M, const VI* x0,
+                                                const VI* x1, PreF&& pre,
+                                                E&& epi) {
+    spmv_rows_pt<NRHS, MT, VI>(M, x0, x1, pre, epi);
   }
   // Working-copy operators (A, A') in the solver precision.  The ALM
   // passes over the caller's operators (A0, A0') read the same stored
@@ -401,9 +421,22 @@ struct Ops {
         const bool lazy = fl & F_LAZY, pcopy = fl & F_PCOPY;
         const double itheta = s.op_a, theta2 = s.op_b;
         const double lt = s.op_c, ls = 1 - s.op_c;
-        auto epi = [&](uint32_t j, double gy, double gz) {
-          const double cj = (double)pc[j];
-          const double zj = (double)Z[j];
+        struct Pre1 {
+          double c, z, iv, l, u, v;
+        };
+        auto pre = [&](uint32_t j) {
+          Pre1 q;
+          q.c = (double)pc[j];
+          q.z = (double)Z[j];
+          q.iv = (double)iv[j];
+          q.l = (double)P(a.p_l)[j];
+          q.u = (double)P(a.p_u)[j];
+          q.v = lng ? (double)vv[j] : 0.0;
+          return q;
+        };
+        auto epi = [&](uint32_t j, double gy, double gz, const Pre1& q) {
+          const double cj = q.c;
+          const double zj = q.z;
           if (pcopy || lazy) {
             const VI xj = X[j];
             if (pcopy) PV[j] = xj;
@@ -411,8 +444,8 @@ struct Ops {
           }
           const double g2 = gy + cj;
           G2[j] = (VI)g2;
-          const double stp = itheta * (double)iv[j];
-          const VI tv = (VI)clamp_lu(j, zj - g2 * stp);
+          const double stp = itheta * q.iv;
+          const VI tv = (VI)minpd(q.u, maxpd(q.l, zj - g2 * stp));
           const double t = (double)tv;
           Zp[j] = tv;
           p[4] = maxpd(p[4], fabs(t));
@@ -421,22 +454,22 @@ struct Ops {
           if (lng) {
             const double d = t - zj;
             p[0] += g2 * d;
-            p[1] += d * d * theta2 * (double)vv[j];
+            p[1] += d * d * theta2 * q.v;
           }
           if (two) {
             const double gg = gz + cj;
             G1[j] = (VI)gg;
             p[5] += gg * (t - zj);
             if (want_pg) {
-              const double d = zj - clamp_lu(j, zj - gg);
+              const double d = zj - minpd(q.u, maxpd(q.l, zj - gg));
               p[6] += d * d;
             }
           }
         };
         if (two)
-          spmv_rows<2>(a.AT, VM(MYV), VM(MZV), epi);
+          spmv_rows_pre<2>(a.AT, VM(MYV), VM(MZV), pre, epi);
         else
-          spmv_rows<1>(a.AT, VM(MYV), nullptr, epi);
+          spmv_rows_pre<1>(a.AT, VM(MYV), nullptr, pre, epi);
         break;
       }
       case OP_PASS2: {  // compute_violation(zp) + approx/vector.inc:249-295
@@ -451,13 +484,23 @@ struct Ops {
         const bool spec = fl & F_SPEC;
         const double th = s.op_a, thn = s.op_b;
         const double sc = 1 - th, nsc = 1 - thn;
-        spmv_rows<1>(a.A, Zp, nullptr, [&](uint32_t i, double ax, double) {
-          const double v = ax - (double)b[i];
-          const VI svv = (VI)((double)w[i] * v);
+        struct Pre2 {
+          double b, w, xv;
+        };
+        auto pre = [&](uint32_t i) {
+          Pre2 q;
+          q.b = b[i];
+          q.w = w[i];
+          q.xv = (double)XV[i];
+          return q;
+        };
+        spmv_rows_pre<1>(a.A, Zp, nullptr, pre, [&](uint32_t i, double ax, double, const Pre2& q) {
+          const double v = ax - q.b;
+          const VI svv = (VI)(q.w * v);
           const double sv = (double)svv;
           ZPV[i] = svv;
           p[0] += sv * v;
-          const VI xv = (VI)(sc * (double)XV[i] + th * sv);
+          const VI xv = (VI)(sc * q.xv + th * sv);
           const VI yv = (VI)(nsc * (double)xv + thn * sv);
           if (spec) {
             SX[i] = xv;

@@ -183,6 +183,13 @@ class CLib:
             ("alm_solve_ex", C.c_int, [vp, sz, dp, sz, dp, sz, vp, dp,
                                        C.POINTER(AlmOptions)]),
             ("acd_last_solve_stats", None, [C.POINTER(SolveStats)]),
+            ("acd_server_make", vp, [sz, sz, sz, C.POINTER(AlmOptions)]),
+            ("acd_server_free", None, [vp]),
+            ("acd_server_submit", C.c_int, [vp, vp, dp, dp, dp,
+                                            C.POINTER(C.c_int), C.c_uint64]),
+            ("acd_server_step", C.c_int, [vp, C.POINTER(C.c_uint64), sz,
+                                          C.POINTER(sz)]),
+            ("acd_server_pending", sz, [vp]),
             ("alm_solve_batch", C.c_int, [sz, C.POINTER(vp), sz,
                                           C.POINTER(dp), C.POINTER(dp),
                                           C.POINTER(dp), C.POINTER(C.c_int),
@@ -373,6 +380,9 @@ class CLib:
         return ([(int(rets[i]), xs[i], ls[i], ds[i]) for i in range(n)],
                 done, info)
 
+    def server(self, hbm_budget, slots=2048, large_rows=50000, **opts):
+        return Server(self, hbm_budget, slots, large_rows, **opts)
+
     def profile(self):
         """{pass name: (launches, microseconds)} of the last solve."""
         cnt = (C.c_uint64 * 32)()
@@ -409,6 +419,56 @@ class CLib:
         buf = C.create_string_buffer(256)
         code = self.lib.acd_last_error(buf, 256)
         return code, buf.value.decode(errors="replace")
+
+
+class Server:
+    """acd_server_* (include/acd_gpu.h) with the request buffers kept alive
+    until completion."""
+
+    def __init__(self, lib, hbm_budget, slots, large_rows, **opts):
+        self.lib = lib
+        o = AlmOptions()
+        lib.lib.acd_alm_default_options(C.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        self.s = lib.lib.acd_server_make(hbm_budget, slots, large_rows,
+                                         C.byref(o))
+        if not self.s:
+            raise RuntimeError("acd_server_make failed: %s" % (lib.last_error(),))
+        self.cap = slots + 8
+        self.live = {}
+
+    def submit(self, tag, alm, x0, lam0):
+        x = f64(x0).copy()
+        lam = f64(lam0).copy()
+        diag = np.full(3, np.nan)
+        ret = C.c_int(-1)
+        self.live[tag] = (x, lam, diag, ret)
+        rc = self.lib.lib.acd_server_submit(self.s, alm, _d(x), _d(lam),
+                                            _d(diag), C.byref(ret), tag)
+        if rc:
+            raise RuntimeError("acd_server_submit failed")
+
+    def step(self):
+        """One round; returns [(tag, ret, x, lam, diag)] completed."""
+        done = (C.c_uint64 * self.cap)()
+        n = C.c_size_t(0)
+        if self.lib.lib.acd_server_step(self.s, done, self.cap, C.byref(n)):
+            raise RuntimeError("acd_server_step failed: %s"
+                               % (self.lib.last_error(),))
+        out = []
+        for i in range(min(n.value, self.cap)):
+            x, lam, diag, ret = self.live.pop(done[i])
+            out.append((int(done[i]), ret.value, x, lam, diag))
+        return out
+
+    def pending(self):
+        return self.lib.lib.acd_server_pending(self.s)
+
+    def close(self):
+        if self.s:
+            self.lib.lib.acd_server_free(self.s)
+            self.s = None
 
 
 _LIB = None

@@ -24,6 +24,10 @@ struct alm {
 };
 
 struct acd_log_ctx { FILE * log; int restart; };
+/* Caller vectors of an ALM object to and from its device LP (aborts on
+ * failure, like the solve entry points). */
+void acd_alm_upload(alm_t * alm, const double * x, const double * lambda);
+void acd_alm_download(alm_t * alm, double * x, double * lambda, size_t niter);
 void acd_log_sink(void * ctx, const struct acd_event * e);
 int acd_approx_upload(approx_t * a);
 void acd_record_stats(const struct acd_solve_result * r);

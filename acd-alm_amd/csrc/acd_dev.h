@@ -204,6 +204,30 @@ int acd_dev_solve_batch(struct acd_dev_lp * const * lps, size_t count,
                         struct acd_solve_result * res, double * done_ms,
                         struct acd_batch_stats * stats);
 
+/* ---- continuous batching (dev_solver.hip) ---- */
+/* A pool of `cap` slots for the batched kernel: LPs are admitted into free
+ * slots between rounds; a round launches the batched kernel `launches`
+ * times over the listed slots (each advancing by up to `budget` inner
+ * iterations) and returns the per-slot done flags (pinned host array). */
+struct acd_dev_pool;
+struct acd_dev_pool * acd_dev_pool_create(uint32_t cap, int precision);
+void acd_dev_pool_free(struct acd_dev_pool *);
+int acd_dev_pool_admit(struct acd_dev_pool *, uint32_t slot,
+                       struct acd_dev_lp *, const struct acd_solve_request *);
+int acd_dev_pool_round(struct acd_dev_pool *, const uint32_t * slots,
+                       uint32_t n, uint32_t launches, uint64_t budget,
+                       uint32_t ** flags);
+int acd_dev_pool_result(struct acd_dev_pool *, uint32_t slot,
+                        struct acd_solve_result *);
+/* Non-blocking single-LP solve on the LP's own stream (no log): begin,
+ * then poll until it returns 1 (done, result filled); 0 = still running,
+ * -1 = error.  Uploads/downloads of the LP's vectors are the caller's. */
+int acd_dev_lp_async_begin(struct acd_dev_lp *,
+                           const struct acd_solve_request *);
+int acd_dev_lp_async_poll(struct acd_dev_lp *, struct acd_solve_result *);
+/* Block until the chunk in flight has finished. */
+int acd_dev_lp_async_wait(struct acd_dev_lp *);
+
 /* ---- randomised coordinate descent (dev_cd.hip, approx/cd.c) ---- */
 /* Works on the caller-order vectors of an acd_dev_lp (x, rhs, weight,
  * linear, lower, upper) and the unpermuted matrix. */

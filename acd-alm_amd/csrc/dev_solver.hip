@@ -174,7 +174,7 @@ template <class VI, class VC, class MT>
 __global__ void __launch_bounds__(64 * kWPB, ACD_BATCH_MIN_BLOCKS)
 k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
         uint64_t budget, uint32_t* done_count, uint64_t* t_done,
-        uint64_t* bytes) {
+        uint64_t* bytes, uint32_t* flags) {
   __shared__ Args la;
   __shared__ State ls;
   __shared__ double red[kWPB][kRes];
@@ -236,6 +236,7 @@ k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
     bytes[lp] += moved;
     if (ls.pc == PC_DONE || ls.error) {
       t_done[lp] = wall_clock64();
+      flags[lp] = 1u;
       atomicAdd(done_count, 1u);
     }
   }
@@ -274,6 +275,12 @@ struct acd_dev_lp {
   hipGraphExec_t graph[kKeys];
   int have_graph[kKeys];
   int pkey;  // precision key the permuted problem was last written in
+  // non-blocking solve (acd_dev_lp_async_*)
+  hipStream_t as;
+  hipEvent_t aev;
+  State* ahs;  // pinned host copy of the state
+  int akey, achunk, abusy;
+  uint64_t alaunches;
 };
 
 namespace {
@@ -551,28 +558,28 @@ int precision_key(acd_dev_lp* lp, int precision) {
 
 void launch_batch(int key, uint32_t nblocks, const Args* args, State* st,
                   const uint32_t* slots, uint64_t budget, uint32_t* done,
-                  uint64_t* t_done, uint64_t* bytes) {
+                  uint64_t* t_done, uint64_t* bytes, uint32_t* flags) {
   const dim3 grid(nblocks), block(64 * kWPB);
   switch (key) {
     case K_MIX_F32:
       k_batch<double, float, float><<<grid, block, 0, S()>>>(
-          args, st, slots, budget, done, t_done, bytes);
+          args, st, slots, budget, done, t_done, bytes, flags);
       break;
     case K_MIX_BF16:
       k_batch<double, float, uint16_t><<<grid, block, 0, S()>>>(
-          args, st, slots, budget, done, t_done, bytes);
+          args, st, slots, budget, done, t_done, bytes, flags);
       break;
     case K_F32:
       k_batch<float, float, float><<<grid, block, 0, S()>>>(
-          args, st, slots, budget, done, t_done, bytes);
+          args, st, slots, budget, done, t_done, bytes, flags);
       break;
     case K_F32_BF16:
       k_batch<float, float, uint16_t><<<grid, block, 0, S()>>>(
-          args, st, slots, budget, done, t_done, bytes);
+          args, st, slots, budget, done, t_done, bytes, flags);
       break;
     default:
       k_batch<double, double, double><<<grid, block, 0, S()>>>(
-          args, st, slots, budget, done, t_done, bytes);
+          args, st, slots, budget, done, t_done, bytes, flags);
       break;
   }
 }
@@ -639,6 +646,9 @@ void acd_dev_lp_free(struct acd_dev_lp* lp) {
   if (!lp) return;
   for (int k = 0; k < kKeys; k++)
     if (lp->have_graph[k]) hipGraphExecDestroy(lp->graph[k]);
+  if (lp->aev) hipEventDestroy(lp->aev);
+  if (lp->ahs) hipHostFree(lp->ahs);
+  if (lp->as) hipStreamDestroy(lp->as);
   for (int v = 0; v < ACD_V_COUNT; v++) acd_dev_free(lp->vec[v]);
   acd_dev_free(lp->r);
   acd_dev_free(lp->p_rhs);
@@ -731,6 +741,7 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   State* d_st = nullptr;
   uint32_t *d_slots = nullptr, *d_count = nullptr;
   uint64_t *d_done = nullptr, *d_bytes = nullptr, *d_t0 = nullptr;
+  uint32_t* d_flags = nullptr;
   int rc = 0;
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
@@ -742,7 +753,8 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
       hipMalloc((void**)&d_count, sizeof(uint32_t)) != hipSuccess ||
       hipMalloc((void**)&d_done, count * sizeof(uint64_t)) != hipSuccess ||
       hipMalloc((void**)&d_bytes, count * sizeof(uint64_t)) != hipSuccess ||
-      hipMalloc((void**)&d_t0, sizeof(uint64_t)) != hipSuccess) {
+      hipMalloc((void**)&d_t0, sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc((void**)&d_flags, count * sizeof(uint32_t)) != hipSuccess) {
     acd_dev_record_error(-1, "batch allocation");
     rc = -1;
   }
@@ -775,6 +787,8 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
         hipMemsetAsync(d_count, 0, sizeof(uint32_t), S()) != hipSuccess ||
         hipMemsetAsync(d_bytes, 0, count * sizeof(uint64_t), S()) !=
             hipSuccess ||
+        hipMemsetAsync(d_flags, 0, count * sizeof(uint32_t), S()) !=
+            hipSuccess ||
         hipStreamSynchronize(S()) != hipSuccess)
       rc = -1;
   }
@@ -789,7 +803,7 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
     for (int round = 0; !rc && active; round++) {
       for (int l = 0; l < 8; l++) {
         launch_batch(key, active, d_args, d_st, d_slots, 32, d_count, d_done,
-                     d_bytes);
+                     d_bytes, d_flags);
         launches++;
       }
       if (hipGetLastError() != hipSuccess ||
@@ -875,12 +889,218 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   hipFree(d_done);
   hipFree(d_bytes);
   hipFree(d_t0);
+  hipFree(d_flags);
   free(h_args);
   free(h_st);
   free(h_slots);
   free(h_done);
   free(h_bytes);
   return rc;
+}
+
+// ---- slot pool for continuous batching ----------------------------------
+struct acd_dev_pool {
+  uint32_t cap;
+  int key;
+  Args* d_args;
+  State* d_st;
+  uint32_t *d_slots, *d_count, *d_flags;
+  uint64_t *d_done, *d_bytes;
+  uint32_t* h_flags;  // pinned
+};
+
+struct acd_dev_pool* acd_dev_pool_create(uint32_t cap, int precision) {
+  if (acd_dev_init() || cap == 0) return nullptr;
+  acd_dev_pool* p = (acd_dev_pool*)calloc(1, sizeof(acd_dev_pool));
+  if (!p) return nullptr;
+  p->cap = cap;
+  // One kernel instantiation for every LP that may arrive: reduced
+  // precision uses FP32 matrix values (BF16 would need every future LP to
+  // be exact there).
+  p->key = precision == ACD_DEV_MIXED ? K_MIX_F32
+           : precision == ACD_DEV_FP32 ? K_F32 : K_F64;
+  if (hipMalloc((void**)&p->d_args, cap * sizeof(Args)) != hipSuccess ||
+      hipMalloc((void**)&p->d_st, cap * sizeof(State)) != hipSuccess ||
+      hipMalloc((void**)&p->d_slots, cap * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void**)&p->d_count, sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void**)&p->d_flags, cap * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void**)&p->d_done, cap * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc((void**)&p->d_bytes, cap * sizeof(uint64_t)) != hipSuccess ||
+      hipHostMalloc((void**)&p->h_flags, cap * sizeof(uint32_t),
+                    hipHostMallocDefault) != hipSuccess) {
+    acd_dev_record_error(-1, "pool allocation");
+    acd_dev_pool_free(p);
+    return nullptr;
+  }
+  hipMemsetAsync(p->d_flags, 0, cap * sizeof(uint32_t), S());
+  hipMemsetAsync(p->d_count, 0, sizeof(uint32_t), S());
+  hipMemsetAsync(p->d_bytes, 0, cap * sizeof(uint64_t), S());
+  hipStreamSynchronize(S());
+  return p;
+}
+
+void acd_dev_pool_free(struct acd_dev_pool* p) {
+  if (!p) return;
+  hipStreamSynchronize(S());
+  hipFree(p->d_args);
+  hipFree(p->d_st);
+  hipFree(p->d_slots);
+  hipFree(p->d_count);
+  hipFree(p->d_flags);
+  hipFree(p->d_done);
+  hipFree(p->d_bytes);
+  if (p->h_flags) hipHostFree(p->h_flags);
+  free(p);
+}
+
+int acd_dev_pool_admit(struct acd_dev_pool* p, uint32_t slot,
+                       struct acd_dev_lp* lp,
+                       const struct acd_solve_request* q) {
+  if (slot >= p->cap) return -1;
+  if (p->key != K_F64) {
+    acd_dev_matrix* pm = const_cast<acd_dev_matrix*>(lp->perm);
+    acd_dev_matrix* om = const_cast<acd_dev_matrix*>(lp->orig);
+    if (acd_dev_sell_make_fp32(&pm->a) || acd_dev_sell_make_fp32(&pm->at) ||
+        acd_dev_sell_make_fp32(&om->a) || acd_dev_sell_make_fp32(&om->at))
+      return -1;
+  }
+  Args a = make_args(lp, p->key);
+  a.ev = nullptr;
+  a.ev_cap = 0;
+  a.st = p->d_st + slot;
+  a.prof = nullptr;
+  a.partials = nullptr;
+  State st;
+  acd_solve_request qi = *q;
+  qi.log = 0;
+  init_state(&st, lp, &qi);
+  const uint32_t zero = 0;
+  if (hipMemcpyAsync(p->d_args + slot, &a, sizeof a, hipMemcpyHostToDevice,
+                     S()) != hipSuccess ||
+      hipMemcpyAsync(p->d_st + slot, &st, sizeof st, hipMemcpyHostToDevice,
+                     S()) != hipSuccess ||
+      hipMemcpyAsync(p->d_flags + slot, &zero, 4, hipMemcpyHostToDevice,
+                     S()) != hipSuccess ||
+      hipStreamSynchronize(S()) != hipSuccess) {
+    acd_dev_record_error(-1, "pool admit");
+    return -1;
+  }
+  return 0;
+}
+
+int acd_dev_pool_round(struct acd_dev_pool* p, const uint32_t* slots,
+                       uint32_t n, uint32_t launches, uint64_t budget,
+                       uint32_t** flags) {
+  if (n > p->cap) return -1;
+  if (n) {
+    if (hipMemcpyAsync(p->d_slots, slots, n * sizeof(uint32_t),
+                       hipMemcpyHostToDevice, S()) != hipSuccess)
+      return -1;
+    for (uint32_t l = 0; l < launches; l++)
+      launch_batch(p->key, n, p->d_args, p->d_st, p->d_slots, budget,
+                   p->d_count, p->d_done, p->d_bytes, p->d_flags);
+  }
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(p->h_flags, p->d_flags, p->cap * sizeof(uint32_t),
+                     hipMemcpyDeviceToHost, S()) != hipSuccess ||
+      hipStreamSynchronize(S()) != hipSuccess) {
+    acd_dev_record_error(-1, "pool round");
+    return -1;
+  }
+  if (flags) *flags = p->h_flags;
+  return 0;
+}
+
+int acd_dev_pool_result(struct acd_dev_pool* p, uint32_t slot,
+                        struct acd_solve_result* r) {
+  State hs;
+  if (slot >= p->cap ||
+      hipMemcpyAsync(&hs, p->d_st + slot, sizeof hs, hipMemcpyDeviceToHost,
+                     S()) != hipSuccess ||
+      hipStreamSynchronize(S()) != hipSuccess)
+    return -1;
+  memset(r, 0, sizeof *r);
+  r->error = hs.error;
+  r->ret = hs.alm_ret;
+  for (int k = 0; k < 3; k++) r->diag[k] = hs.alm_diag[k];
+  r->inner_iterations = hs.inner_total;
+  r->outer_iterations = hs.outer;
+  r->kkt = hs.kkt;
+  return 0;
+}
+
+// ---- non-blocking single-LP solve on its own stream ------------------------
+// begin: state upload, graph, first chunk; poll: if the last chunk has
+// finished, either report completion (1) or launch the next chunk (0).
+int acd_dev_lp_async_begin(struct acd_dev_lp* lp,
+                           const struct acd_solve_request* q) {
+  if (!lp->as &&
+      hipStreamCreateWithFlags(&lp->as, hipStreamNonBlocking) != hipSuccess)
+    return -1;
+  if (!lp->aev && hipEventCreateWithFlags(&lp->aev, hipEventDisableTiming) !=
+                      hipSuccess)
+    return -1;
+  if (!lp->ahs && hipHostMalloc((void**)&lp->ahs, sizeof(State),
+                                hipHostMallocDefault) != hipSuccess)
+    return -1;
+  const int key = precision_key(lp, q->precision);
+  if (key < 0 || build_graph(lp, key)) return -1;
+  acd_solve_request qi = *q;
+  qi.log = 0;
+  init_state(lp->ahs, lp, &qi);
+  lp->akey = key;
+  lp->achunk = 1;
+  lp->alaunches = 0;
+  Args a = make_args(lp, key);
+  hipStream_t st = lp->as;
+  if (hipMemcpyAsync(lp->st, lp->ahs, sizeof(State), hipMemcpyHostToDevice,
+                     st) != hipSuccess ||
+      hipMemsetAsync(lp->prof, 0, sizeof(Prof), st) != hipSuccess)
+    return -1;
+  k_ctl<<<1, 256, 0, st>>>(a, (uint32_t)lp->blocks);
+  if (hipGraphLaunch(lp->graph[key], st) != hipSuccess ||
+      hipMemcpyAsync(lp->ahs, lp->st, sizeof(State), hipMemcpyDeviceToHost,
+                     st) != hipSuccess ||
+      hipEventRecord(lp->aev, st) != hipSuccess)
+    return -1;
+  lp->alaunches = 2 * kPairs + 1;
+  lp->abusy = 1;
+  return 0;
+}
+
+int acd_dev_lp_async_wait(struct acd_dev_lp* lp) {
+  if (!lp->abusy) return 0;
+  return hipEventSynchronize(lp->aev) == hipSuccess ? 0 : -1;
+}
+
+int acd_dev_lp_async_poll(struct acd_dev_lp* lp, struct acd_solve_result* r) {
+  if (!lp->abusy) return -1;
+  const hipError_t q = hipEventQuery(lp->aev);
+  if (q == hipErrorNotReady) return 0;
+  if (q != hipSuccess) return -1;
+  State* hs = lp->ahs;
+  if (hs->pc == PC_DONE || hs->error) {
+    memset(r, 0, sizeof *r);
+    r->error = hs->error;
+    r->ret = hs->alm_ret;
+    for (int k = 0; k < 3; k++) r->diag[k] = hs->alm_diag[k];
+    r->inner_iterations = hs->inner_total;
+    r->outer_iterations = hs->outer;
+    r->launches = lp->alaunches;
+    r->kkt = hs->kkt;
+    lp->abusy = 0;
+    return 1;
+  }
+  hipStream_t st = lp->as;
+  for (int c = 0; c < lp->achunk; c++)
+    if (hipGraphLaunch(lp->graph[lp->akey], st) != hipSuccess) return -1;
+  lp->alaunches += (uint64_t)lp->achunk * 2 * kPairs;
+  if (lp->achunk < 16) lp->achunk *= 2;
+  if (hipMemcpyAsync(hs, lp->st, sizeof(State), hipMemcpyDeviceToHost, st) !=
+          hipSuccess ||
+      hipEventRecord(lp->aev, st) != hipSuccess)
+    return -1;
+  return 0;
 }
 
 }  // extern "C"

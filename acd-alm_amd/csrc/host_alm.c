@@ -137,7 +137,7 @@ int alm_solve_ex(alm_t * alm, size_t niter, double * x, size_t nvars,
         return r.ret;
 }
 
-static void upload_alm(alm_t * alm, const double * x, const double * lambda)
+void acd_alm_upload(alm_t * alm, const double * x, const double * lambda)
 {
         approx_t * a = alm->approx;
         struct acd_dev_lp * dev = a->dev;
@@ -151,7 +151,7 @@ static void upload_alm(alm_t * alm, const double * x, const double * lambda)
                 acd_fail("alm_solve: upload");
 }
 
-static void download_alm(alm_t * alm, double * x, double * lambda,
+void acd_alm_download(alm_t * alm, double * x, double * lambda,
                          size_t niter)
 {
         approx_t * a = alm->approx;
@@ -180,7 +180,7 @@ int alm_solve_batch(size_t count, alm_t * const * alms, size_t niter,
         struct acd_solve_result * res = calloc(count, sizeof(*res));
         if (lps == NULL || res == NULL) acd_fail("alm_solve_batch: malloc");
         for (size_t i = 0; i < count; i++) {
-                upload_alm(alms[i], x[i], lambda[i]);
+                acd_alm_upload(alms[i], x[i], lambda[i]);
                 lps[i] = alms[i]->approx->dev;
         }
         struct acd_solve_request q;
@@ -199,7 +199,7 @@ int alm_solve_batch(size_t count, alm_t * const * alms, size_t niter,
         if (rc) acd_fail("alm_solve_batch: device solve failed");
         for (size_t i = 0; i < count; i++) {
                 acd_check_result(&res[i], 0, "alm_solve_batch");
-                download_alm(alms[i], x[i], lambda[i], niter);
+                acd_alm_download(alms[i], x[i], lambda[i], niter);
                 if (OUT_diagnosis != NULL && OUT_diagnosis[i] != NULL)
                         for (int k = 0; k < 3; k++)
                                 OUT_diagnosis[i][k] = res[i].diag[k];

@@ -182,6 +182,78 @@ def batch_leg(lib, count, prec, stop_kkt, steps=1, fp64_sample=8):
     }
 
 
+def serve_leg(lib, rate, seconds, stop_kkt, prec=1, seed=9,
+              large_frac=0.01, budget_gb=64.0):
+    """Request loop (continuous batching): Poisson arrivals at `rate` per
+    second for `seconds`; 99% LPs built like the batched workload, 1% large
+    (m=100,000, n=300,000, 8 nonzeros per column).  Each request's ALM
+    object is made before the clock starts (its data device-resident); the
+    latency of a request runs from its arrival time to the step that
+    retires it."""
+    from acdalm import instances, kkt
+    rng = np.random.default_rng(seed)
+    t, arrivals = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            break
+        arrivals.append(t)
+    large = rng.random(len(arrivals)) < large_frac
+    ms = np.rint(np.exp(rng.uniform(np.log(50), np.log(2000), len(arrivals))))
+    ns = np.rint(rng.uniform(2.0, 3.0, len(arrivals)) * ms)
+    seeds = rng.integers(0, 2**62, size=len(arrivals))
+    t0 = time.perf_counter()
+    lps = [instances.random_lp(100000, 300000, 8, int(sd)) if lg else
+           instances.random_lp(int(m), int(n), 5, int(sd))
+           for lg, m, n, sd in zip(large, ms, ns, seeds)]
+    mats = [lib.make_matrix(lp, 0) for lp in lps]
+    alms = [lib.alm_make(m, lp) for m, lp in zip(mats, lps)]
+    setup = time.perf_counter() - t0
+    srv = lib.server(int(budget_gb * 2**30), slots=2048, large_rows=50000,
+                     precision=prec, stop_kkt=stop_kkt * (0.9 if prec else 1.0))
+    lat = np.zeros(len(lps))
+    results = {}
+    nxt = 0
+    start = time.perf_counter()
+    while nxt < len(lps) or srv.pending():
+        now = time.perf_counter() - start
+        while nxt < len(lps) and arrivals[nxt] <= now:
+            srv.submit(nxt, alms[nxt], np.zeros(lps[nxt].ncols),
+                       np.zeros(lps[nxt].nrows))
+            nxt += 1
+        done = srv.step()
+        now = time.perf_counter() - start
+        for tag, ret, x, lam, diag in done:
+            lat[tag] = (now - arrivals[tag]) * 1e3
+            results[tag] = (ret, x, lam)
+        if not done and nxt < len(lps) and not srv.pending():
+            time.sleep(max(0.0, min(1e-3, arrivals[nxt] - now)))
+    end = time.perf_counter() - start
+    srv.close()
+    kk = [kkt.kkt(lp, results[i][1], results[i][2])["kkt"]
+          for i, lp in enumerate(lps)]
+    for a in alms:
+        lib.alm_free(a)
+    for m in mats:
+        lib.free_matrix(m)
+    small = ~large
+    return {
+        "workload": "Poisson arrivals at %g/s for %g s: %d requests (%d "
+                    "large m=100000 n=300000)" % (rate, seconds, len(lps),
+                                                   int(large.sum())),
+        "solves_per_s": len(lps) / end, "offered_per_s": rate,
+        "makespan_s": end,
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "small_p50_ms": float(np.percentile(lat[small], 50)) if small.any() else None,
+        "small_p99_ms": float(np.percentile(lat[small], 99)) if small.any() else None,
+        "large_p50_ms": float(np.percentile(lat[large], 50)) if large.any() else None,
+        "kkt_max": float(max(kk)),
+        "all_converged": all(r[0] == 0 for r in results.values()),
+        "setup_s": setup,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,8 +268,12 @@ def main():
                          "this (0: the reference's absolute rule)")
     ap.add_argument("--outer", type=int, default=1000,
                     help="ALM outer-iteration cap (the reference uses 1000)")
-    ap.add_argument("--workload", choices=["setcover", "batch"],
+    ap.add_argument("--workload", choices=["setcover", "batch", "serve"],
                     default="setcover")
+    ap.add_argument("--rate", type=float, default=2000.0,
+                    help="serve: offered requests per second")
+    ap.add_argument("--seconds", type=float, default=1.0,
+                    help="serve: length of the arrival window")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--no-batch", action="store_true",
                     help="setcover workload: skip the batched-solve leg")
@@ -215,6 +291,22 @@ def main():
     lib = capi.load()
     L = lib.lib
     prec = {"mixed": 1, "fp64": 0}[args.precision]
+
+    if args.workload == "serve":
+        r = serve_leg(lib, args.rate, args.seconds, args.stop_kkt or 1e-6,
+                      prec=prec)
+        print(json.dumps({
+            "metric": METRIC, "value": r["solves_per_s"], "unit": "solves/s",
+            "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": r["makespan_s"] * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic (seeded generator)",
+            "config": {"workload": r["workload"], "precision": args.precision,
+                       "stop": "relative KKT <= %g" % (args.stop_kkt or 1e-6),
+                       "parallelism": "single GPU"},
+            "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+            "roofline": None, "cpu_baseline": None, "serve": r}), flush=True)
+        return
 
     if args.workload == "batch":
         L.acd_synchronize()

@@ -109,6 +109,33 @@ int alm_solve_batch(size_t count, alm_t * const * alms, size_t niter,
                     double * OUT_done_ms, uint64_t * OUT_inner,
                     struct acd_batch_info * OUT_info);
 
+/* ---- request loop (continuous batching) ---- */
+typedef struct acd_server acd_server_t;
+/* A request loop on the current device.  hbm_budget bounds the summed
+ * device working set of the requests running at once (a request larger
+ * than the budget runs alone); `slots` is the number of small LPs that run
+ * together in the batched kernel (0: 2048); LPs with at least `large_rows`
+ * rows run the multi-workgroup solver on their own stream instead (up to
+ * 4 at once).  Options as for alm_solve_ex (niter = 1000). */
+acd_server_t * acd_server_make(size_t hbm_budget, size_t slots,
+                               size_t large_rows,
+                               const struct acd_alm_options * options);
+void acd_server_free(acd_server_t *);
+/* Queue one solve of `alm` from x / lambda (caller-owned, in/out, like
+ * alm_solve; they and OUT_diagnosis (double[3] or NULL) and OUT_ret must
+ * stay valid until the request completes).  The ALM object must not be
+ * used elsewhere meanwhile. */
+int acd_server_submit(acd_server_t *, alm_t * alm, double * x,
+                      double * lambda, double * OUT_diagnosis, int * OUT_ret,
+                      uint64_t tag);
+/* One round: admit queued requests that fit, advance running ones, retire
+ * finished ones.  Writes up to `cap` completed tags (cap >= slots + 4
+ * never drops one) and their count to *OUT_ndone. */
+int acd_server_step(acd_server_t *, uint64_t * OUT_done, size_t cap,
+                    size_t * OUT_ndone);
+/* Requests queued or running. */
+size_t acd_server_pending(const acd_server_t *);
+
 /* Restore the row weights (1) and penalised linear term (0) that alm_make
  * starts from, so the next alm_solve repeats a fresh solve; the original
  * keeps them across alm_solve calls. */

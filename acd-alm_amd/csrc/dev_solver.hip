@@ -439,8 +439,10 @@ bool use_graphs() {
 }
 
 int run_to_completion(acd_dev_lp* lp, int key, State* hs,
-                      acd_solve_result* res, acd_event_sink sink, void* ctx) {
-  if (use_graphs() && build_graph(lp, key)) return -1;
+                      acd_solve_result* res, acd_event_sink sink, void* ctx,
+                      bool graphs = true) {
+  graphs = graphs && use_graphs();
+  if (graphs && build_graph(lp, key)) return -1;
   Args a = make_args(lp, key);
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
@@ -457,7 +459,7 @@ int run_to_completion(acd_dev_lp* lp, int key, State* hs,
   int chunk = 1;
   while (!rc) {
     for (int c = 0; c < chunk && !rc; c++) {
-      if (use_graphs()) {
+      if (graphs) {
         if (hipGraphLaunch(lp->graph[key], S()) != hipSuccess) rc = -1;
       } else {
         for (int i = 0; i < kPairs; i++) {
@@ -677,7 +679,9 @@ int acd_dev_approx_update(struct acd_dev_lp* lp) {
   State hs;
   init_state(&hs, lp, &q);
   acd_solve_result res;
-  const int rc = run_to_completion(lp, K_F64, &hs, &res, nullptr, nullptr);
+  // Two ops: launched directly, no graph to capture.
+  const int rc =
+      run_to_completion(lp, K_F64, &hs, &res, nullptr, nullptr, false);
   lp->pkey = K_F64;
   return rc;
 }
@@ -695,7 +699,8 @@ int acd_dev_solve(struct acd_dev_lp* lp, const struct acd_solve_request* q,
     memset(&u, 0, sizeof u);
     State hu;
     init_state(&hu, lp, &u);
-    if (run_to_completion(lp, key, &hu, nullptr, nullptr, nullptr)) return -1;
+    if (run_to_completion(lp, key, &hu, nullptr, nullptr, nullptr, false))
+      return -1;
     lp->pkey = key;
   }
   State hs;

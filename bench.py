@@ -292,6 +292,13 @@ def main():
     L = lib.lib
     prec = {"mixed": 1, "fp64": 0}[args.precision]
 
+    if args.copy_check:
+        src = L.acd_device_alloc(args.copy_check)
+        dst = L.acd_device_alloc(args.copy_check)
+        L.acd_copy_device(dst, src, args.copy_check)
+        L.acd_device_free(src)
+        L.acd_device_free(dst)
+
     if args.workload == "serve":
         r = serve_leg(lib, args.rate, args.seconds, args.stop_kkt or 1e-6,
                       prec=prec)
@@ -327,13 +334,6 @@ def main():
             "roofline": dict(b["roofline"], bound="hbm", traffic=None),
             "cpu_baseline": None, "batch": b, "wall_s": total}), flush=True)
         return
-
-    if args.copy_check:
-        src = L.acd_device_alloc(args.copy_check)
-        dst = L.acd_device_alloc(args.copy_check)
-        L.acd_copy_device(dst, src, args.copy_check)
-        L.acd_device_free(src)
-        L.acd_device_free(dst)
 
     t0 = time.perf_counter()
     lp = instances.set_cover()

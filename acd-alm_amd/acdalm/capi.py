@@ -28,6 +28,9 @@ _libc.fopen.restype = vp
 _libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
 _libc.fclose.argtypes = [vp]
 _libc.fflush.argtypes = [vp]
+_libc.fdopen.restype = vp
+_libc.fdopen.argtypes = [C.c_int, C.c_char_p]
+_libc.setvbuf.argtypes = [vp, C.c_char_p, C.c_int, C.c_size_t]
 
 
 class AlmOptions(C.Structure):
@@ -66,13 +69,19 @@ def f64(a, n=None):
 
 
 class LogCapture:
-    """A C FILE* writing to a temporary file; .text after close()."""
+    """A C FILE* writing to a temporary file; .text after close().  With
+    enabled="stderr" the log goes straight to the process's stderr (the
+    solver writes each line as it is produced) and .text stays empty."""
 
     def __init__(self, enabled=True):
         self.enabled = enabled
         self.fp = None
         self.text = ""
-        if enabled:
+        self.path = None
+        if enabled == "stderr":
+            self.fp = _libc.fdopen(os.dup(2), b"w")
+            _libc.setvbuf(self.fp, None, 2, 0)  # unbuffered
+        elif enabled:
             fd, self.path = tempfile.mkstemp(prefix="acdlog")
             os.close(fd)
             self.fp = _libc.fopen(self.path.encode(), b"w")
@@ -82,9 +91,10 @@ class LogCapture:
             _libc.fflush(self.fp)
             _libc.fclose(self.fp)
             self.fp = None
-            with open(self.path) as f:
-                self.text = f.read()
-            os.unlink(self.path)
+            if self.path:
+                with open(self.path) as f:
+                    self.text = f.read()
+                os.unlink(self.path)
         return self.text
 
 

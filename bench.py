@@ -53,6 +53,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+class _StdoutToStderr:
+    """Sends fd 1 to stderr while the work runs (the solver prints a
+    newline on stdout after each restart, as the reference does), so the
+    JSON line is the only thing on stdout."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        C.CDLL(None).fflush(None)
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
+
+
+def emit(line):
+    print(json.dumps(line), flush=True)
+
+
 def spmv_leg(lib, mat, lp, reps=50):
     """Stand-alone SpMV (y = A x and y = A' x) on the solver's matrix,
     timed with HIP events on the library stream."""
@@ -266,9 +289,12 @@ def main():
     ap.add_argument("--stop-kkt", type=float, default=1e-6,
                     help="stop once the relative KKT residual is at most "
                          "this (0: the reference's absolute rule)")
+    ap.add_argument("--log", action="store_true",
+                    help="stream the ALM log (one line per outer iteration) "
+                         "to stderr")
     ap.add_argument("--outer", type=int, default=1000,
                     help="ALM outer-iteration cap (the reference uses 1000)")
-    ap.add_argument("--workload", choices=["setcover", "batch", "serve"],
+    ap.add_argument("--workload", choices=["setcover", "mcf", "batch", "serve"],
                     default="setcover")
     ap.add_argument("--rate", type=float, default=2000.0,
                     help="serve: offered requests per second")
@@ -302,7 +328,7 @@ def main():
     if args.workload == "serve":
         r = serve_leg(lib, args.rate, args.seconds, args.stop_kkt or 1e-6,
                       prec=prec)
-        print(json.dumps({
+        return ({
             "metric": METRIC, "value": r["solves_per_s"], "unit": "solves/s",
             "n_gpus": 1, "steps": 1, "warmup": 0,
             "ms_per_step": r["makespan_s"] * 1e3,
@@ -312,8 +338,7 @@ def main():
                        "stop": "relative KKT <= %g" % (args.stop_kkt or 1e-6),
                        "parallelism": "single GPU"},
             "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
-            "roofline": None, "cpu_baseline": None, "serve": r}), flush=True)
-        return
+            "roofline": None, "cpu_baseline": None, "serve": r})
 
     if args.workload == "batch":
         L.acd_synchronize()
@@ -321,7 +346,7 @@ def main():
         b = batch_leg(lib, args.batch_count, prec, args.stop_kkt,
                       steps=args.steps)
         total = time.perf_counter() - t
-        print(json.dumps({
+        return ({
             "metric": METRIC, "value": b["solves_per_s"], "unit": "solves/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": b["device_ms_per_batch"],
@@ -332,11 +357,17 @@ def main():
                        "parallelism": "single GPU"},
             "p50_ms": b["p50_ms"], "p99_ms": b["p99_ms"],
             "roofline": dict(b["roofline"], bound="hbm", traffic=None),
-            "cpu_baseline": None, "batch": b, "wall_s": total}), flush=True)
-        return
+            "cpu_baseline": None, "batch": b, "wall_s": total})
 
     t0 = time.perf_counter()
-    lp = instances.set_cover()
+    if args.workload == "mcf":
+        lp = instances.multicommodity()
+        workload = ("multicommodity flow LP: 100x100 grid, 39,600 arcs, 64 "
+                    "commodities, m=%d n=%d, +-1 A" % (lp.nrows, lp.ncols))
+    else:
+        lp = instances.set_cover()
+        workload = ("set-cover LP relaxation m=200000 n=1000000+200000 "
+                    "slack, 0/1 A, 3-8 nnz/column")
     mat = lib.make_matrix(lp, 0)
     alm = lib.alm_make(mat, lp)
     x0 = np.zeros(lp.ncols)
@@ -350,9 +381,9 @@ def main():
     def one_solve():
         L.acd_alm_reset(alm)
         t = time.perf_counter()
-        rc, x, lam, diag, _ = lib.alm_solve_ex(alm, args.outer, x0, lam0,
-                                               precision=prec,
-                                               stop_kkt=args.stop_kkt)
+        rc, x, lam, diag, _ = lib.alm_solve_ex(
+            alm, args.outer, x0, lam0, log="stderr" if args.log else False,
+            precision=prec, stop_kkt=args.stop_kkt)
         return time.perf_counter() - t, rc, x, lam, diag
 
     for w in range(args.warmup):
@@ -428,9 +459,8 @@ def main():
         "ms_per_step": total / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded set-cover generator)",
-        "config": {"workload": "set-cover LP relaxation m=200000 n=1000000"
-                               "+200000 slack, 0/1 A, 3-8 nnz/column",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": workload,
                    "precision": args.precision,
                    "stop": "relative KKT <= %g" % args.stop_kkt
                            if args.stop_kkt > 0 else "reference rule",
@@ -448,8 +478,10 @@ def main():
         "cpu_baseline": cpu,
         "setup_s": setup_s,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 if __name__ == "__main__":
-    main()
+    with _StdoutToStderr():
+        result = main()
+    emit(result)

@@ -43,7 +43,8 @@ class AlmOptions(C.Structure):
 class BatchInfo(C.Structure):
     """struct acd_batch_info (include/acd_gpu.h)."""
     _fields_ = [("device_ms", C.c_double), ("launches", C.c_uint64),
-                ("inner_iterations", C.c_uint64), ("alg_bytes", C.c_double)]
+                ("inner_iterations", C.c_uint64), ("alg_bytes", C.c_double),
+                ("waves_per_lp", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -210,6 +211,7 @@ class CLib:
             ("acd_spmv_variant_time", C.c_int, [vp, C.c_int, vp, vp, C.c_int,
                                                 C.c_int, C.POINTER(C.c_double)]),
             ("acd_spmv_set_variant", C.c_int, [C.c_int]),
+            ("acd_spmv_autotune", C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
             ("acd_last_solve_profile", None, [C.POINTER(C.c_uint64),
                                               C.POINTER(C.c_double), C.c_int]),
             ("acd_last_solve_bytes", None, [C.POINTER(C.c_double),

@@ -52,6 +52,8 @@ struct acd_sell {
         float * val32;         /* device [nent] or NULL (mixed precision) */
         uint16_t * val16;      /* device [nent] bf16 bits, or NULL */
         uint32_t * rowlen;     /* device [nrows]: entries per row */
+        int32_t tuned;         /* SpMV variant + 1 picked by the autotuner,
+                                  0 = library default */
 };
 
 /* ---- preprocessing (dev_preprocess.hip) ---- */
@@ -86,6 +88,12 @@ int acd_dev_sell_make_bf16(struct acd_sell *);
 /* y = op x on device pointers (op = A' when transpose). */
 int acd_dev_spmv(const struct acd_dev_matrix *, int transpose,
                  double * y, const double * x);
+enum { ACD_DEV_SPMV_VARIANTS = 9 };
+/* Times every SpMV variant (unroll x slices per wavefront x load kind) on
+ * both operators of m, `reps` products each, and keeps the fastest per
+ * operator; out[0], out[1] receive the choices for A and A'.  Variants
+ * differ only in scheduling: every row is summed in the same order. */
+int acd_dev_spmv_autotune(struct acd_dev_matrix * m, int reps, int * out);
 int acd_dev_spmv2(const struct acd_dev_matrix *, int transpose,
                   double * y0, double * y1, const double * x0,
                   const double * x1);
@@ -198,6 +206,7 @@ struct acd_batch_stats {
         uint64_t launches;
         uint64_t inner_iterations; /* summed over the LPs */
         double alg_bytes;          /* algorithmic bytes, summed */
+        int waves_per_lp;          /* autotuned workgroup size / 64 */
 };
 int acd_dev_solve_batch(struct acd_dev_lp * const * lps, size_t count,
                         const struct acd_solve_request * q,

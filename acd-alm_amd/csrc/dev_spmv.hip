@@ -219,7 +219,8 @@ extern "C" {
 
 int acd_dev_spmv(const struct acd_dev_matrix* m, int transpose, double* y,
                  const double* x) {
-  return launch1(pick(m, transpose), y, x, g_default_variant);
+  const struct acd_sell* op = pick(m, transpose);
+  return launch1(op, y, x, op->tuned ? op->tuned - 1 : g_default_variant);
 }
 
 int acd_dev_spmv2(const struct acd_dev_matrix* m, int transpose, double* y0,
@@ -266,6 +267,42 @@ int acd_dev_spmv_time_variant(const struct acd_dev_matrix* m, int transpose,
 int acd_dev_spmv_time(const struct acd_dev_matrix* m, int transpose, double* y,
                       const double* x, int reps, double* OUT_ms) {
   return acd_dev_spmv_time_variant(m, transpose, y, x, reps, -1, OUT_ms);
+}
+
+int acd_dev_spmv_autotune(struct acd_dev_matrix* m, int reps, int* out) {
+  if (reps < 1) reps = 1;
+  const size_t nx = (size_t)(m->ncols > m->nrows ? m->ncols : m->nrows) + 1;
+  double *x = nullptr, *y = nullptr;
+  if (hipMalloc((void**)&x, nx * 8) != hipSuccess ||
+      hipMalloc((void**)&y, nx * 8) != hipSuccess) {
+    hipFree(x);
+    acd_dev_record_error(-1, "autotune buffers");
+    return -1;
+  }
+  hipMemsetAsync(x, 0, nx * 8, S());
+  int rc = 0;
+  for (int tr = 0; tr < 2 && !rc; tr++) {
+    struct acd_sell* op = tr ? &m->at : &m->a;
+    int best = g_default_variant;
+    double best_ms = 0;
+    for (int v = 0; v < ACD_DEV_SPMV_VARIANTS && !rc; v++) {
+      double ms = 0;
+      if (acd_dev_spmv_time_variant(m, tr, y, x, reps, v, &ms)) {
+        rc = -1;
+        break;
+      }
+      if (v == 0 || ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+    op->tuned = best + 1;
+    if (out) out[tr] = best;
+  }
+  hipStreamSynchronize(S());
+  hipFree(x);
+  hipFree(y);
+  return rc;
 }
 
 int acd_dev_spmv_set_default_variant(int variant) {

@@ -56,6 +56,11 @@ int acd_spmv_variant_time(const sparse_matrix_t * a, int transpose,
                                          variant, OUT_ms);
 }
 
+int acd_spmv_autotune(sparse_matrix_t * a, int reps, int * OUT_variants)
+{
+        return acd_dev_spmv_autotune(&a->dev, reps, OUT_variants);
+}
+
 int acd_spmv_set_variant(int variant)
 {
         if (variant < 0 || variant >= ACD_SPMV_VARIANTS) return -1;

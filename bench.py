@@ -78,9 +78,14 @@ def emit(line):
 
 def spmv_leg(lib, mat, lp, reps=50):
     """Stand-alone SpMV (y = A x and y = A' x) on the solver's matrix,
-    timed with HIP events on the library stream."""
+    timed with HIP events on the library stream, after the autotuner has
+    picked the variant (unroll, rows per wavefront, load kind) per
+    operator."""
     L = lib.lib
-    out = {}
+    tuned = (C.c_int * 2)()
+    if L.acd_spmv_autotune(mat, 10, tuned):
+        raise RuntimeError("SpMV autotune failed: %s" % (lib.last_error(),))
+    out = {"variant_A": int(tuned[0]), "variant_AT": int(tuned[1])}
     for tr in (0, 1):
         ny, nx = (lp.ncols, lp.nrows) if tr else (lp.nrows, lp.ncols)
         dx = L.acd_device_alloc(8 * nx)

@@ -51,6 +51,13 @@ int acd_spmv_variant_time(const sparse_matrix_t *, int transpose,
                           int variant, double * OUT_ms_per_launch);
 int acd_spmv_set_variant(int variant);
 enum { ACD_SPMV_VARIANTS = 9 };
+/* Autotuner: times every variant on both operators of the matrix (`reps`
+ * products each) and keeps the fastest per operator for
+ * sparse_matrix_multiply and acd_spmv_device.  Variants only schedule
+ * differently (unroll depth, rows per wavefront, load kind): every row is
+ * summed in the same order, so results do not change.  OUT_variants
+ * (optional, int[2]) receives the choices for A and A'. */
+int acd_spmv_autotune(sparse_matrix_t *, int reps, int * OUT_variants);
 
 /* Stored entries of the operator (padding to the slice length included). */
 uint64_t acd_matrix_stored_entries(const sparse_matrix_t *, int transpose);
@@ -92,6 +99,8 @@ struct acd_batch_info {
         uint64_t launches;           /* batch kernel launches */
         uint64_t inner_iterations;   /* summed over the LPs */
         double alg_bytes;            /* algorithmic bytes moved, summed */
+        int waves_per_lp;            /* wavefronts per LP workgroup, picked
+                                        by a timed trial on a sample */
 };
 /* alm_solve_ex on `count` independent ALM objects at once: one workgroup
  * per LP, every LP on the device together, repeated launches of 32 inner

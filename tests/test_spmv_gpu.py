@@ -114,3 +114,26 @@ def test_dense_agreement_like_reference_test(gpu):
         assert spmv_scaled_error(y, d @ x, lp.rows, lp.cols, lp.vals, x,
                                  17) < 1e-13
         gpu.free_matrix(m)
+
+
+def test_autotuned_variants_give_identical_products(gpu, oracle):
+    """The autotuner only changes scheduling: every variant it may keep
+    sums each row in the same order, so products stay bit-identical."""
+    import ctypes
+    lp = I.set_cover(3000, 12000, 4)
+    mg, mo = gpu.make_matrix(lp, 1), oracle.make_matrix(lp, 1)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(lp.ncols)
+    y = rng.standard_normal(lp.nrows)
+    for v in range(9):
+        gpu.lib.acd_spmv_set_variant(v)
+        assert bit_equal(gpu.multiply(mg, x, 0), oracle.multiply(mo, x, 0)), v
+        assert bit_equal(gpu.multiply(mg, y, 1), oracle.multiply(mo, y, 1)), v
+    gpu.lib.acd_spmv_set_variant(1)
+    tuned = (ctypes.c_int * 2)()
+    assert gpu.lib.acd_spmv_autotune(mg, 3, tuned) == 0
+    assert 0 <= tuned[0] < 9 and 0 <= tuned[1] < 9
+    assert bit_equal(gpu.multiply(mg, x, 0), oracle.multiply(mo, x, 0))
+    assert bit_equal(gpu.multiply(mg, y, 1), oracle.multiply(mo, y, 1))
+    gpu.free_matrix(mg)
+    oracle.free_matrix(mo)

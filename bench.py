@@ -145,7 +145,43 @@ def cpu_baseline(lp, inner_iterations, iters, threads):
                           inner_iterations))}
 
 
-def batch_leg(lib, count, prec, stop_kkt, steps=1, fp64_sample=8):
+def batch_cpu_baseline(lps, sample=32, procs=16):
+    """The reference's CPU path on the batched workload, LP-parallel (one
+    LP per process, pool = NULL, as its thread pool cannot be shared by
+    concurrent solves): `sample` LPs spread over the size-sorted batch,
+    `procs` at a time; solves/s = sample / wall time.  It stops on the
+    reference's absolute rule (its only rule)."""
+    ref = os.path.join(HERE, "oracle", "_ref", "ref_bench")
+    if not os.path.exists(ref):
+        return {"value": None, "error": "reference build absent"}
+    order = sorted(range(len(lps)), key=lambda i: lps[i].nnz)
+    pick = [order[(k * len(order)) // sample] for k in range(sample)]
+    with tempfile.TemporaryDirectory() as td:
+        paths = []
+        for k, i in enumerate(pick):
+            path = os.path.join(td, "lp%d.lpb" % k)
+            lps[i].write_binary(path)
+            paths.append(path)
+        t = time.perf_counter()
+        running, fails = [], 0
+        for path in paths:
+            while len(running) >= procs:
+                fails += running.pop(0).wait() != 0
+            running.append(subprocess.Popen(
+                [ref, path, "1", "alm"], stdout=subprocess.DEVNULL,
+                stderr=subprocess.DEVNULL))
+        for p in running:
+            fails += p.wait() != 0
+        wall = time.perf_counter() - t
+    return {"value": sample / wall, "unit": "solves/s", "cores": procs,
+            "kind": "reference", "failed": fails,
+            "sample": ("%d LPs spread over the size-sorted batch, one per "
+                       "process, %d processes at a time, reference stopping "
+                       "rule (%.1f s wall)" % (sample, procs, wall))}
+
+
+def batch_leg(lib, count, prec, stop_kkt, steps=1, fp64_sample=8,
+              cpu=True):
     """Batched solve of `count` independent LPs (m log-uniform in [50, 2000],
     n = 2-3 m, 5 nonzeros per column, random +-U[0.5, 2] values; seeded),
     all device-resident, to the relative KKT target.  Also times the plain
@@ -190,6 +226,12 @@ def batch_leg(lib, count, prec, stop_kkt, steps=1, fp64_sample=8):
         lib.alm_free(a)
     for m in mats:
         lib.free_matrix(m)
+    cpu_b = None
+    if cpu:
+        try:
+            cpu_b = batch_cpu_baseline(lps, procs=min(16, os.cpu_count() or 1))
+        except Exception as e:  # reported, never silently replaced
+            cpu_b = {"value": None, "error": str(e)[-300:]}
     rate = count * steps / (dev_ms * 1e-3)
     gbs = alg / (dev_ms * 1e-3) / 1e9
     return {
@@ -206,6 +248,7 @@ def batch_leg(lib, count, prec, stop_kkt, steps=1, fp64_sample=8):
         "fp64_single_lp_solves_per_s": single,
         "fp64_single_lp_sample": fp64_sample,
         "vs_fp64_single_lp": rate / single,
+        "cpu_baseline": cpu_b,
         "setup_s": setup,
     }
 
@@ -349,7 +392,7 @@ def main():
         L.acd_synchronize()
         t = time.perf_counter()
         b = batch_leg(lib, args.batch_count, prec, args.stop_kkt,
-                      steps=args.steps)
+                      steps=args.steps, cpu=not args.no_cpu)
         total = time.perf_counter() - t
         return ({
             "metric": METRIC, "value": b["solves_per_s"], "unit": "solves/s",
@@ -362,7 +405,7 @@ def main():
                        "parallelism": "single GPU"},
             "p50_ms": b["p50_ms"], "p99_ms": b["p99_ms"],
             "roofline": dict(b["roofline"], bound="hbm", traffic=None),
-            "cpu_baseline": None, "batch": b, "wall_s": total})
+            "cpu_baseline": b["cpu_baseline"], "batch": b, "wall_s": total})
 
     t0 = time.perf_counter()
     if args.workload == "mcf":
@@ -445,7 +488,8 @@ def main():
     batch = None
     if not args.no_batch:
         try:
-            batch = batch_leg(lib, args.batch_count, 1, args.stop_kkt or 1e-6)
+            batch = batch_leg(lib, args.batch_count, 1, args.stop_kkt or 1e-6,
+                              cpu=not args.no_cpu)
         except Exception as e:  # reported, never silently replaced
             batch = {"error": str(e)[-300:]}
     cpu = None

@@ -195,6 +195,9 @@ void acd_dev_last_profile(uint64_t * counts, double * us, int n);
  * the control kernel's start (the step kernel's duration plus the control
  * launch latency), and algorithmic bytes moved. */
 void acd_dev_last_profile_ex(double * step_us, double * bytes, int n);
+/* Control kernel time split over the last solve: microseconds in the
+ * loads, the combine and the decision, and the number of calls. */
+void acd_dev_last_ctl_phases(double * us4);
 
 /* ---- batched solves (dev_solver.hip) ---- */
 /* Runs request q (ALM mode) on every LP, one workgroup per LP, in repeated

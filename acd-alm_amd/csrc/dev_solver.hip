@@ -39,6 +39,7 @@ uint64_t g_prof_count[32];
 double g_prof_us[32];
 double g_prof_step_us[32];
 double g_prof_bytes[32];
+double g_ctl_us[4];
 
 struct GridG {
   uint32_t lane, gwave, nwaves;
@@ -108,6 +109,7 @@ __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
       dst[i] = src[i];
   }
   __syncthreads();
+  const uint64_t t_loaded = wall_clock64();
   const uint32_t mm = ls.op_maxmask;
   const int32_t op = ls.op;
   const bool ran = op != OP_NONE;
@@ -140,9 +142,16 @@ __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
   }
   __syncthreads();
   if (threadIdx.x == 0 && live) {
+    const uint64_t t_comb = wall_clock64();
     Ctl c{ls, a, res};
     c.advance();
     ls.t_last = wall_clock64();
+    if (ran) {
+      a.prof->ctl_phase[0] += t_loaded - t_entry;
+      a.prof->ctl_phase[1] += t_comb - t_loaded;
+      a.prof->ctl_phase[2] += ls.t_last - t_comb;
+      a.prof->ctl_phase[3] += 1;
+    }
   }
   __syncthreads();
   if (live) {
@@ -517,6 +526,9 @@ int run_to_completion(acd_dev_lp* lp, int key, State* hs,
           khz > 0 ? 1e3 * (double)pf.step_ticks[i] / khz : 0.0;
       g_prof_bytes[i] = (double)pf.op_bytes[i];
     }
+    for (int i = 0; i < 3; i++)
+      g_ctl_us[i] = khz > 0 ? 1e3 * (double)pf.ctl_phase[i] / khz : 0.0;
+    g_ctl_us[3] = (double)pf.ctl_phase[3];
   }
   if (res) {
     res->error = hs->error ? hs->error : (rc ? ACD_ERR_LAUNCH : 0);
@@ -612,6 +624,10 @@ void acd_dev_last_profile(uint64_t* counts, double* us, int n) {
     counts[i] = g_prof_count[i];
     us[i] = g_prof_us[i];
   }
+}
+
+void acd_dev_last_ctl_phases(double* us4) {
+  for (int i = 0; i < 4; i++) us4[i] = g_ctl_us[i];
 }
 
 void acd_dev_last_profile_ex(double* step_us, double* bytes, int n) {

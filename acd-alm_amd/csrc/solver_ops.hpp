@@ -175,6 +175,7 @@ struct Prof {
   uint64_t op_ticks[32];        // ticks from the op's issue to its control
   uint64_t step_ticks[32];      // ticks from step kernel start to control
   uint64_t op_bytes[32];        // algorithmic bytes moved per op
+  uint64_t ctl_phase[4];        // control kernel: loads, combine, decide
 };
 
 struct Args {
@@ -321,11 +322,31 @@ struct Ops {
           v0 = w0; v1 = w1; v2 = w2; v3 = w3;
         }
       }
-      for (; k < len; k++, e += 64) {
-        const double vv = mval<MT2>(M, e);
-        const uint32_t cc = M.col[e];
-        a0 = a0 + vv * (double)x0[cc];
-        if (NRHS == 2) a1 = a1 + vv * (double)x1[cc];
+      // The last len % 4 entries as one batch: their loads and gathers
+      // are issued together, the adds stay in entry order.
+      const uint32_t rem = len - k;
+      if (rem) {
+        const bool h1 = rem > 1, h2 = rem > 2;
+        const uint32_t c0 = M.col[e];
+        const uint32_t c1 = h1 ? M.col[e + 64] : 0u;
+        const uint32_t c2 = h2 ? M.col[e + 128] : 0u;
+        const double v0 = mval<MT2>(M, e);
+        const double v1 = h1 ? mval<MT2>(M, e + 64) : 0.0;
+        const double v2 = h2 ? mval<MT2>(M, e + 128) : 0.0;
+        const double p0 = x0[c0];
+        const double p1 = h1 ? (double)x0[c1] : 0.0;
+        const double p2 = h2 ? (double)x0[c2] : 0.0;
+        a0 = a0 + v0 * p0;
+        if (h1) a0 = a0 + v1 * p1;
+        if (h2) a0 = a0 + v2 * p2;
+        if (NRHS == 2) {
+          const double q0 = x1[c0];
+          const double q1 = h1 ? (double)x1[c1] : 0.0;
+          const double q2 = h2 ? (double)x1[c2] : 0.0;
+          a1 = a1 + v0 * q0;
+          if (h1) a1 = a1 + v1 * q1;
+          if (h2) a1 = a1 + v2 * q2;
+        }
       }
       if (live) epi(r, a0, a1, pv);
       if (nsl >= M.nslices) break;

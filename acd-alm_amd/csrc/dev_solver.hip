@@ -340,14 +340,21 @@ Args make_args(acd_dev_lp* lp, int key) {
 }
 
 // Enough wavefronts for every slice of the largest operator, capped at
-// what the chip keeps resident (grid-stride beyond that).
+// what the chip keeps resident at once (grid-stride beyond that).
 int pick_blocks(const acd_dev_matrix* orig, const acd_dev_matrix* perm) {
   uint32_t slices = perm->at.nslices;
   if (perm->a.nslices > slices) slices = perm->a.nslices;
   if (orig->at.nslices > slices) slices = orig->at.nslices;
   if (orig->a.nslices > slices) slices = orig->a.nslices;
   int want = (int)((slices + kWPB - 1) / kWPB);
-  int cap = 8 * acd_dev_cu_count();
+  // One residency round: the step kernel runs 4 waves per SIMD (its
+  // register count), i.e. 4 workgroups of 4 waves per CU; measured on the
+  // set-cover LP, 4 per CU beat 2, 6 and 8 per CU.
+  int cap = 4 * acd_dev_cu_count();
+  if (const char* e = getenv("ACD_STEP_BLOCKS")) {  // experiments
+    const int v = atoi(e);
+    if (v >= 64 && v <= (int)kMaxBlocks) cap = v;
+  }
   if (want > 64) want = (want + 7) / 8 * 8;  // whole XCD rounds
   if (cap < 64) cap = 64;
   if (cap > (int)kMaxBlocks) cap = (int)kMaxBlocks;

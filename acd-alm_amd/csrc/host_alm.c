@@ -198,12 +198,15 @@ int alm_solve_batch(size_t count, alm_t * const * alms, size_t niter,
         int rc = acd_dev_solve_batch(lps, count, &q, res, OUT_done_ms, &st);
         if (rc) acd_fail("alm_solve_batch: device solve failed");
         for (size_t i = 0; i < count; i++) {
-                acd_check_result(&res[i], 0, "alm_solve_batch");
+                /* A failed LP (non-finite iterate, positive model
+                 * estimate) is reported in its own return code; the
+                 * other LPs of the batch are unaffected. */
                 acd_alm_download(alms[i], x[i], lambda[i], niter);
                 if (OUT_diagnosis != NULL && OUT_diagnosis[i] != NULL)
                         for (int k = 0; k < 3; k++)
                                 OUT_diagnosis[i][k] = res[i].diag[k];
-                if (OUT_ret != NULL) OUT_ret[i] = res[i].ret;
+                if (OUT_ret != NULL)
+                        OUT_ret[i] = res[i].error ? -res[i].error : res[i].ret;
                 if (OUT_inner != NULL) OUT_inner[i] = res[i].inner_iterations;
         }
         if (OUT_info != NULL) {

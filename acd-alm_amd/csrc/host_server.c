@@ -139,13 +139,14 @@ size_t acd_server_pending(const acd_server_t * s)
         return s->qlen + s->nactive + s->nlarge;
 }
 
+/* A request whose solve failed (non-finite iterate, positive model
+ * estimate) completes with a negative return code; the loop goes on. */
 static void finish(struct request * r, const struct acd_solve_result * res)
 {
-        acd_check_result(res, 0, "acd_server");
         acd_alm_download(r->alm, r->x, r->lambda, 1);
         if (r->diag != NULL)
                 for (int k = 0; k < 3; k++) r->diag[k] = res->diag[k];
-        if (r->ret != NULL) *r->ret = res->ret;
+        if (r->ret != NULL) *r->ret = res->error ? -res->error : res->ret;
 }
 
 /* FIFO admission while the head fits: a free slot (small) or a large lane

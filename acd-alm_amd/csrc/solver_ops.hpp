@@ -205,8 +205,13 @@ static_assert(sizeof(Args) % 8 == 0, "Args is copied as 64-bit words");
 __device__ __forceinline__ double maxpd(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double minpd(double a, double b) { return a < b ? a : b; }
 
+// Max-combines propagate NaN (the reference's maxpd chains do), so a
+// non-finite iterate reaches the unboundedness test instead of being
+// dropped by fmax and leaving the solve to spin on NaN comparisons.
 __device__ __forceinline__ double combine(double a, double b, bool is_max) {
-  return is_max ? fmax(a, b) : a + b;
+  if (!is_max) return a + b;
+  if (a != a || b != b) return a + b;
+  return fmax(a, b);
 }
 
 __device__ __forceinline__ double wave_reduce(double v, bool is_max) {
@@ -1009,6 +1014,7 @@ struct Ctl {
         break;
       }
       case PC_I_LONG1: {  // approx/approx.c:343-376
+        if (!(res[4] < HUGE_VAL)) return fail(ACD_ERR_UNBOUNDED);
         s.t_expected = res[0] + .5 * res[1];
         s.t_cz = res[2];
         s.t_czp = res[3];

@@ -93,7 +93,14 @@ int alm_solve_ex(alm_t *, size_t niter, double * x, size_t nvars,
                  double * lambda, size_t nconstraints, FILE * log,
                  double * OUT_diagnosis, const struct acd_alm_options *);
 
-/* ---- batched solves ---- */
+/* ---- batched solves and the request loop ---- */
+/* A batched or served LP whose solve fails gets the negated code as its
+ * return value (the single-LP entry points abort instead, as the
+ * reference's asserts do). */
+enum acd_solve_error {
+        ACD_SOLVE_UNBOUNDED = 1,   /* an iterate is not finite */
+        ACD_SOLVE_ESTIMATE = 2     /* a long step's model estimate > 0 */
+};
 struct acd_batch_info {
         double device_ms;            /* device time of the whole batch */
         uint64_t launches;           /* batch kernel launches */
@@ -107,7 +114,8 @@ struct acd_batch_info {
  * iterations until all are done (no extension of the reference; a loop
  * of alm_solve_ex calls gives the same results up to reduction order).
  * x[i], lambda[i] (in/out) and OUT_diagnosis[i] (double[3], may be NULL)
- * are per LP; OUT_ret[i] receives alm_solve's return code.  OUT_done_ms
+ * are per LP; OUT_ret[i] receives alm_solve's return code, or
+ * -enum acd_solve_error.  OUT_done_ms
  * and OUT_inner (optional, count entries) receive each LP's completion
  * time from the batch start and its inner iteration count.  Each LP's result does not depend on the other LPs in the
  * batch. */

@@ -105,3 +105,19 @@ def test_batch_kkt_stopping_rule(gpu):
     for lp, (ret, x, lam, diag) in zip(lps, out):
         assert ret == 0
         assert kkt(lp, x, lam)["kkt"] <= 1e-6
+
+
+def test_a_failing_lp_is_reported_and_the_others_are_unaffected(gpu):
+    """A NaN cost makes that LP's iterate non-finite: its return code is
+    -ACD_SOLVE_UNBOUNDED while the other LPs solve as if alone."""
+    lps = I.batch(4, seed=9, mmin=30, mmax=100)
+    bad = I.random_lp(40, 100, 4, 10)
+    bad.linear = bad.linear.copy()
+    bad.linear[7] = np.nan
+    print("[batch] failing LP", flush=True)
+    res, _, _ = batch(gpu, lps[:2] + [bad] + lps[2:], 0)
+    assert res[2][0] == -1
+    ok = [r for i, r in enumerate(res) if i != 2]
+    for lp, r in zip(lps, ok):
+        alone, _, _ = batch(gpu, [lp], 0)
+        assert same(r, alone[0])

@@ -78,3 +78,14 @@ def test_small_budget_admits_one_at_a_time(gpu):
     narrow = serve(gpu, lps, budget=1, precision=0)
     for i in range(len(lps)):
         assert same(wide[i], narrow[i]), i
+
+
+def test_a_failing_request_completes_with_an_error(gpu):
+    lps = problems()[:3]
+    bad = I.random_lp(40, 100, 4, 10)
+    bad.linear = bad.linear.copy()
+    bad.linear[7] = np.nan
+    print("[serve] failing request", flush=True)
+    out = serve(gpu, lps + [bad], precision=0)
+    assert out[3][0] == -1
+    assert all(out[i][0] == 0 for i in range(3))

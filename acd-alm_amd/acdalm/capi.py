@@ -37,7 +37,8 @@ class AlmOptions(C.Structure):
     """struct acd_alm_options (include/acd_gpu.h)."""
     _fields_ = [("stop_violation", C.c_double), ("stop_pg", C.c_double),
                 ("precision_floor", C.c_double), ("min_delta", C.c_double),
-                ("precision", C.c_int), ("stop_kkt", C.c_double)]
+                ("precision", C.c_int), ("engine", C.c_int),
+                ("stop_kkt", C.c_double)]
 
 
 class BatchInfo(C.Structure):

@@ -84,6 +84,7 @@ void acd_alm_default_options(struct acd_alm_options * o)
         o->precision_floor = 1e-7;
         o->min_delta = 1e-11;
         o->precision = ACD_FP64;
+        o->engine = ACD_ENGINE_AUTO;
         o->stop_kkt = 0;
 }
 
@@ -119,7 +120,15 @@ int alm_solve_ex(alm_t * alm, size_t niter, double * x, size_t nvars,
         q.stop_kkt = o.stop_kkt;
         struct acd_solve_result r;
         struct acd_log_ctx ctx = {log, 0};
-        int rc = acd_dev_solve(dev, &q, &r, acd_log_sink, &ctx);
+        int small = m + n <= 4000;
+        int rc;
+        if (o.engine == ACD_ENGINE_WORKGROUP
+            || (o.engine == ACD_ENGINE_AUTO && log == NULL && small)) {
+                q.log = 0;
+                rc = acd_dev_solve_batch(&dev, 1, &q, &r, NULL, NULL);
+        } else {
+                rc = acd_dev_solve(dev, &q, &r, acd_log_sink, &ctx);
+        }
         acd_check_result(&r, rc, "alm_solve");
         acd_record_stats(&r);
         /* x, lambda, and the state the original leaves in its approx

@@ -63,6 +63,8 @@ def test_result_is_the_same_alone_and_in_the_mix(gpu, precision):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_batch_agrees_with_single_solver(gpu, precision):
+    """Against the multi-workgroup solver (engine=1), whose reductions run
+    in another order."""
     lps = I.batch(8, seed=7, mmin=30, mmax=200)
     print(f"[batch] vs single precision={precision}", flush=True)
     res, _, _ = batch(gpu, lps, precision)
@@ -71,7 +73,7 @@ def test_batch_agrees_with_single_solver(gpu, precision):
         a = gpu.alm_make(m, lp)
         rs, xs, ls, ds, _ = gpu.alm_solve_ex(a, 1000, np.zeros(lp.ncols),
                                              np.zeros(lp.nrows),
-                                             precision=precision)
+                                             precision=precision, engine=1)
         gpu.alm_free(a)
         gpu.free_matrix(m)
         assert ret == rs == 0
@@ -121,3 +123,19 @@ def test_a_failing_lp_is_reported_and_the_others_are_unaffected(gpu):
     for lp, r in zip(lps, ok):
         alone, _, _ = batch(gpu, [lp], 0)
         assert same(r, alone[0])
+
+
+def test_single_small_lp_routes_to_the_workgroup_solver(gpu):
+    """alm_solve_ex picks the one-workgroup solver for small LPs solved
+    without a log; its result is the batch-of-one result."""
+    lp = I.batch(1, seed=12, mmin=60, mmax=80)[0]
+    m = gpu.make_matrix(lp, 0)
+    a = gpu.alm_make(m, lp)
+    r_auto = gpu.alm_solve_ex(a, 1000, np.zeros(lp.ncols), np.zeros(lp.nrows))
+    gpu.lib.acd_alm_reset(a)
+    r_wg = gpu.alm_solve_ex(a, 1000, np.zeros(lp.ncols), np.zeros(lp.nrows),
+                            engine=2)
+    gpu.alm_free(a)
+    gpu.free_matrix(m)
+    alone, _, _ = batch(gpu, [lp], 0)
+    assert same(r_auto[:4], r_wg[:4]) and same(r_auto[:4], alone[0])

@@ -204,12 +204,14 @@ void acd_dev_last_ctl_phases(double * us4);
  * launches of 32 inner iterations until all are done.  res[i] receives
  * each LP's result; done_ms[i] (optional) the device time from the batch
  * start to that LP's completion. */
+#define ACD_DEV_BATCH_CLASSES 8
 struct acd_batch_stats {
         double device_ms;          /* HIP events around all launches */
         uint64_t launches;
         uint64_t inner_iterations; /* summed over the LPs */
         double alg_bytes;          /* algorithmic bytes, summed */
-        int waves_per_lp;          /* autotuned workgroup size / 64 */
+        /* workgroup size / 64 used for each size class (0: no LP) */
+        int waves_per_lp[ACD_DEV_BATCH_CLASSES];
 };
 int acd_dev_solve_batch(struct acd_dev_lp * const * lps, size_t count,
                         const struct acd_solve_request * q,

@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #include <algorithm>
 #include "acd_dev.h"
 #include "solver_ops.hpp"
@@ -164,29 +165,38 @@ __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
 
 // ---------------------------------------------------------------------------
 // Batched solves: one workgroup per LP runs the same ops and control
-// continuation with the workgroup as the group (its 4 waves split the
-// slices) and __syncthreads as the barrier, with the LP's argument and
-// state records in LDS.  A launch advances every unfinished LP by up to
-// `budget` inner iterations (and at most kBatchOps ops), then saves its
-// state, so launches can be repeated, and LPs added between them, until
-// every LP is done.
+// continuation with the workgroup as the group and __syncthreads as the
+// barrier, with the LP's argument and state records in LDS.  A launch
+// advances every unfinished LP by up to `budget` inner iterations (and at
+// most kBatchOps ops), then saves its state, so launches can be repeated,
+// and LPs added between them, until every LP is done.
+//
+// Every op is computed as kVWaves virtual waves: virtual wave v takes
+// slices v, v + kVWaves, ... and reduces its lanes' partials on its own;
+// the kVWaves results are combined in order.  A workgroup of WPB physical
+// waves (1, 2 or 4) runs virtual waves w, w + WPB, ... one after the
+// other, so the sums, and the LP's whole trajectory, are the same for
+// every WPB: the workgroup size is a pure speed choice.
 constexpr uint32_t kBatchOps = 1u << 16;
+constexpr uint32_t kVWaves = 4;
 
 struct BlockG {
   uint32_t lane, gwave, nwaves;
 };
 
-#ifndef ACD_BATCH_MIN_BLOCKS
-#define ACD_BATCH_MIN_BLOCKS 1
-#endif
+// Two waves per SIMD for every workgroup size (the kernel needs ~240
+// VGPRs; without the bound the 1- and 2-wave variants take the whole
+// register file and run one wave per SIMD).
 template <class VI, class VC, class MT, int WPB>
-__global__ void __launch_bounds__(64 * WPB, ACD_BATCH_MIN_BLOCKS)
+__global__ void __launch_bounds__(64 * WPB)
+__attribute__((amdgpu_waves_per_eu(2)))
 k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
         uint64_t budget, uint32_t* done_count, uint64_t* t_done,
         uint64_t* bytes, uint32_t* flags) {
+  static_assert(kVWaves % WPB == 0, "virtual waves per physical wave");
   __shared__ Args la;
   __shared__ State ls;
-  __shared__ double red[WPB][kRes];
+  __shared__ double red[kVWaves][kRes];
   __shared__ double res[kRes];
   const uint32_t lp = slots[blockIdx.x];
   const uint32_t tid = threadIdx.x;
@@ -203,25 +213,31 @@ k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
   if (ls.pc == PC_DONE || ls.error) return;
   const uint64_t stop_at = ls.inner_total + budget;
   uint64_t moved = 0;
-  BlockG g;
-  g.lane = tid & 63;
-  g.gwave = tid >> 6;
-  g.nwaves = WPB;
   for (uint32_t guard = 0; guard < kBatchOps; guard++) {
     if (ls.op != OP_NONE) {
-      Ops<BlockG, VI, VC, MT> ops(g, la, ls);
-      ops.run();
       const uint32_t mm = ls.op_maxmask;
+      for (uint32_t v = tid >> 6; v < kVWaves; v += WPB) {
+        // Reload the LDS records in each pass rather than holding them in
+        // registers across passes (which would spill).
+        asm volatile("" ::: "memory");
+        BlockG g;
+        g.lane = tid & 63;
+        g.gwave = v;
+        g.nwaves = kVWaves;
+        Ops<BlockG, VI, VC, MT> ops(g, la, ls);
+        ops.run();
 #pragma unroll
-      for (int k = 0; k < kRes; k++) {
-        const double v = wave_reduce(ops.p[k], (mm >> k) & 1);
-        if (g.lane == 0) red[g.gwave][k] = v;
+        for (int k = 0; k < kRes; k++) {
+          const double r = wave_reduce(ops.p[k], (mm >> k) & 1);
+          if (g.lane == 0) red[v][k] = r;
+        }
       }
       __syncthreads();
       if (tid < kRes) {
         const bool mx = (mm >> tid) & 1;
         double acc = red[0][tid];
-        for (int w = 1; w < WPB; w++) acc = combine(acc, red[w][tid], mx);
+        for (uint32_t w = 1; w < kVWaves; w++)
+          acc = combine(acc, red[w][tid], mx);
         res[tid] = acc;
       }
       __syncthreads();
@@ -577,50 +593,218 @@ int precision_key(acd_dev_lp* lp, int precision) {
   return bf ? K_MIX_BF16 : K_MIX_F32;
 }
 
+// The batch kernel's buffers, shared by every launch of one batch or pool.
+struct BatchBufs {
+  const Args* args;
+  State* st;
+  uint32_t* done;
+  uint64_t* t_done;
+  uint64_t* bytes;
+  uint32_t* flags;
+};
+
 template <int WPB>
-void launch_batch_w(int key, uint32_t nblocks, const Args* args, State* st,
-                    const uint32_t* slots, uint64_t budget, uint32_t* done,
-                    uint64_t* t_done, uint64_t* bytes, uint32_t* flags) {
+void launch_batch_w(int key, uint32_t nblocks, const uint32_t* slots,
+                    uint64_t budget, const BatchBufs& b, hipStream_t st) {
   const dim3 grid(nblocks), block(64 * WPB);
+#define ACD_BATCH_ARGS \
+  b.args, b.st, slots, budget, b.done, b.t_done, b.bytes, b.flags
   switch (key) {
     case K_MIX_F32:
-      k_batch<double, float, float, WPB><<<grid, block, 0, S()>>>(
-          args, st, slots, budget, done, t_done, bytes, flags);
+      k_batch<double, float, float, WPB><<<grid, block, 0, st>>>(
+          ACD_BATCH_ARGS);
       break;
     case K_MIX_BF16:
-      k_batch<double, float, uint16_t, WPB><<<grid, block, 0, S()>>>(
-          args, st, slots, budget, done, t_done, bytes, flags);
+      k_batch<double, float, uint16_t, WPB><<<grid, block, 0, st>>>(
+          ACD_BATCH_ARGS);
       break;
     case K_F32:
-      k_batch<float, float, float, WPB><<<grid, block, 0, S()>>>(
-          args, st, slots, budget, done, t_done, bytes, flags);
+      k_batch<float, float, float, WPB><<<grid, block, 0, st>>>(
+          ACD_BATCH_ARGS);
       break;
     case K_F32_BF16:
-      k_batch<float, float, uint16_t, WPB><<<grid, block, 0, S()>>>(
-          args, st, slots, budget, done, t_done, bytes, flags);
+      k_batch<float, float, uint16_t, WPB><<<grid, block, 0, st>>>(
+          ACD_BATCH_ARGS);
       break;
     default:
-      k_batch<double, double, double, WPB><<<grid, block, 0, S()>>>(
-          args, st, slots, budget, done, t_done, bytes, flags);
+      k_batch<double, double, double, WPB><<<grid, block, 0, st>>>(
+          ACD_BATCH_ARGS);
       break;
   }
+#undef ACD_BATCH_ARGS
 }
 
-// Waves per LP workgroup: 2, 4 or 8.
-void launch_batch(int key, int wpb, uint32_t nblocks, const Args* args,
-                  State* st, const uint32_t* slots, uint64_t budget,
-                  uint32_t* done, uint64_t* t_done, uint64_t* bytes,
-                  uint32_t* flags) {
-  if (wpb == 2)
-    launch_batch_w<2>(key, nblocks, args, st, slots, budget, done, t_done,
-                      bytes, flags);
-  else if (wpb == 8)
-    launch_batch_w<8>(key, nblocks, args, st, slots, budget, done, t_done,
-                      bytes, flags);
+// Waves per LP workgroup: 1, 2 or 4 (the same results for each).
+void launch_batch(int key, int wpb, uint32_t nblocks, const uint32_t* slots,
+                  uint64_t budget, const BatchBufs& b, hipStream_t st) {
+  if (wpb == 1)
+    launch_batch_w<1>(key, nblocks, slots, budget, b, st);
+  else if (wpb == 2)
+    launch_batch_w<2>(key, nblocks, slots, budget, b, st);
   else
-    launch_batch_w<4>(key, nblocks, args, st, slots, budget, done, t_done,
-                      bytes, flags);
+    launch_batch_w<4>(key, nblocks, slots, budget, b, st);
 }
+
+// ---- workgroup-size groups ------------------------------------------------
+// LPs are put in size classes by their wider operator's slice count
+// (class k: at most 2^k slices of 64 rows; the last class holds the rest),
+// and each class runs with the workgroup size of the current
+// configuration.  LPs of one workgroup size share a launch; the (at most
+// three) launches of a round run on their own streams so that they
+// overlap.
+constexpr int kClasses = ACD_DEV_BATCH_CLASSES;
+constexpr int kWpbChoices = 3;
+constexpr int kWpbOf[kWpbChoices] = {1, 2, 4};
+
+// The tuner's candidate configurations: workgroup size (index into
+// kWpbOf) per size class.  Fewer waves per LP cost an LP more time per
+// op (its slices run in more passes) but leave more of the machine to
+// the others; 4 waves is the latency-optimal size.
+constexpr int kConfigs = 5;
+constexpr uint8_t kConfig[kConfigs][kClasses] = {
+    {2, 2, 2, 2, 2, 2, 2, 2},  // 4 waves
+    {1, 1, 1, 2, 2, 2, 2, 2},  // 2 waves up to 4 slices, 4 above
+    {1, 1, 1, 1, 2, 2, 2, 2},  // 2 waves up to 8 slices, 4 above
+    {1, 1, 1, 1, 1, 2, 2, 2},  // 2 waves up to 16 slices, 4 above
+    {0, 0, 1, 1, 2, 2, 2, 2},  // 1 wave up to 2 slices, 2 up to 8, then 4
+};
+
+int size_class(const acd_dev_lp* lp) {
+  const uint32_t sl = std::max(lp->perm->a.nslices, lp->perm->at.nslices);
+  int c = 0;
+  while (c < kClasses - 1 && (1u << c) < sl) c++;
+  return c;
+}
+
+// Configuration picked by the last tuned batch in this process (batches
+// too small to tune on and the serving pool use it).
+int g_config = 0;
+
+int forced_choice() {
+  const char* e = getenv("ACD_BATCH_WAVES");
+  if (!e) return -1;
+  const int w = atoi(e);
+  for (int i = 0; i < kWpbChoices; i++)
+    if (kWpbOf[i] == w) return i;
+  return -1;
+}
+
+double now_s() {
+  timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+int lp_choice(int cls, int forced, int cfg) {
+  return forced >= 0 ? forced : kConfig[cfg][cls];
+}
+
+struct GroupStreams {
+  hipStream_t s[kWpbChoices];
+  hipEvent_t fork, join[kWpbChoices];
+  bool ok;
+};
+
+GroupStreams* group_streams() {
+  static GroupStreams gs = {};
+  static bool init = false;
+  if (!init) {
+    init = true;
+    gs.ok = hipEventCreateWithFlags(&gs.fork, hipEventDisableTiming) ==
+            hipSuccess;
+    for (int i = 0; i < kWpbChoices && gs.ok; i++)
+      gs.ok = hipStreamCreateWithFlags(&gs.s[i], hipStreamNonBlocking) ==
+                  hipSuccess &&
+              hipEventCreateWithFlags(&gs.join[i], hipEventDisableTiming) ==
+                  hipSuccess;
+  }
+  return gs.ok ? &gs : nullptr;
+}
+
+// One round: `launches` launches for each non-empty group, the groups on
+// their own streams, ordered after and joined back into the library
+// stream.  A group with a captured graph replays it instead.
+struct Group {
+  uint32_t n;          // LPs in the group
+  uint32_t* d_slots;   // its slot list (device)
+  hipGraphExec_t exec; // captured `launches` launches, if any
+  bool have_exec;
+};
+
+int run_round(int key, Group* grp, uint32_t launches, uint64_t budget,
+              const BatchBufs& b, bool graphs) {
+  GroupStreams* gs = group_streams();
+  if (!gs) return -1;
+  if (hipEventRecord(gs->fork, S()) != hipSuccess) return -1;
+  for (int i = 0; i < kWpbChoices; i++) {
+    Group& g = grp[i];
+    if (!g.n) continue;
+    hipStream_t st = gs->s[i];
+    if (hipStreamWaitEvent(st, gs->fork, 0) != hipSuccess) return -1;
+    if (graphs && !g.have_exec) {
+      hipGraph_t graph;
+      if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) !=
+          hipSuccess)
+        return -1;
+      for (uint32_t l = 0; l < launches; l++)
+        launch_batch(key, kWpbOf[i], g.n, g.d_slots, budget, b, st);
+      if (hipStreamEndCapture(st, &graph) != hipSuccess) return -1;
+      const hipError_t e =
+          hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (e != hipSuccess) return -1;
+      g.have_exec = true;
+    }
+    if (graphs) {
+      if (hipGraphLaunch(g.exec, st) != hipSuccess) return -1;
+    } else {
+      for (uint32_t l = 0; l < launches; l++)
+        launch_batch(key, kWpbOf[i], g.n, g.d_slots, budget, b, st);
+    }
+    if (hipEventRecord(gs->join[i], st) != hipSuccess ||
+        hipStreamWaitEvent(S(), gs->join[i], 0) != hipSuccess)
+      return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+void drop_graphs(Group* grp) {
+  for (int i = 0; i < kWpbChoices; i++)
+    if (grp[i].have_exec) {
+      hipGraphExecDestroy(grp[i].exec);
+      grp[i].have_exec = false;
+    }
+}
+
+// Split `live` (in launch order) into the three workgroup-size groups by
+// choice[i] (index into kWpbOf) and upload the lists, group 0 first, into
+// d_slots.  Order inside a group is kept.
+int upload_groups(const uint32_t* live, uint32_t n, const uint8_t* choice,
+                  uint32_t* h_tmp, uint32_t* d_slots, Group* grp) {
+  drop_graphs(grp);
+  uint32_t k = 0;
+  for (int i = 0; i < kWpbChoices; i++) {
+    grp[i].d_slots = d_slots + k;
+    uint32_t c = 0;
+    for (uint32_t j = 0; j < n; j++)
+      if (choice[live[j]] == i) h_tmp[k + c++] = live[j];
+    grp[i].n = c;
+    k += c;
+  }
+  if (n && hipMemcpyAsync(d_slots, h_tmp, n * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, S()) != hipSuccess)
+    return -1;
+  return 0;
+}
+
+// Batches of at least kTuneMinLps LPs tune on their own first rounds
+// (each candidate configuration runs one round, the fastest is kept) and
+// again whenever the number of unfinished LPs has halved, down to
+// kTuneMinLps; smaller batches use the last tuned configuration.
+constexpr size_t kTuneMinLps = 192;
+constexpr double kTuneMargin = 0.95;
+constexpr uint32_t kRoundLaunches = 8;
+constexpr uint64_t kBatchBudget = 32;
 
 }  // namespace
 
@@ -780,8 +964,11 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   Args* h_args = (Args*)calloc(count, sizeof(Args));
   State* h_st = (State*)calloc(count, sizeof(State));
   uint32_t* h_slots = (uint32_t*)malloc(count * sizeof(uint32_t));
+  uint32_t* h_tmp = (uint32_t*)malloc(count * sizeof(uint32_t));
   uint64_t* h_done = (uint64_t*)calloc(count, sizeof(uint64_t));
   uint64_t* h_bytes = (uint64_t*)calloc(count, sizeof(uint64_t));
+  uint8_t* cls = (uint8_t*)malloc(count);
+  uint8_t* choice = (uint8_t*)malloc(count);
   Args* d_args = nullptr;
   State* d_st = nullptr;
   uint32_t *d_slots = nullptr, *d_count = nullptr;
@@ -791,7 +978,8 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
-  if (!h_args || !h_st || !h_slots || !h_done || !h_bytes ||
+  if (!h_args || !h_st || !h_slots || !h_tmp || !h_done || !h_bytes ||
+      !cls || !choice ||
       hipMalloc((void**)&d_args, count * sizeof(Args)) != hipSuccess ||
       hipMalloc((void**)&d_st, count * sizeof(State)) != hipSuccess ||
       hipMalloc((void**)&d_slots, count * sizeof(uint32_t)) != hipSuccess ||
@@ -803,6 +991,15 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
     acd_dev_record_error(-1, "batch allocation");
     rc = -1;
   }
+  // Workgroup size per LP: forced (ACD_BATCH_WAVES=1|2|4), or the
+  // configuration the tuner picks on this batch's own rounds.
+  const int forced = forced_choice();
+  const bool tune = forced < 0 && count >= kTuneMinLps;
+  const bool tune_log = getenv("ACD_BATCH_TUNE_LOG") != nullptr;
+  int cfg = tune ? 0 : g_config, first_cfg = -1;
+  int trial = tune ? 0 : -1;  // configuration on trial, -1: none
+  double trial_s[kConfigs] = {};
+  uint32_t tuned_at = (uint32_t)count;
   uint64_t launches = 0;
   if (!rc) {
     for (size_t i = 0; i < count; i++) {
@@ -816,6 +1013,8 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
       qi.log = 0;
       init_state(&h_st[i], lps[i], &qi);
       h_slots[i] = (uint32_t)i;
+      cls[i] = (uint8_t)size_class(lps[i]);
+      choice[i] = (uint8_t)lp_choice(cls[i], forced, cfg);
     }
     // Largest LPs first: workgroups are dispatched in launch order, so the
     // long solves start at once instead of trailing the batch.
@@ -827,8 +1026,6 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
                        hipMemcpyHostToDevice, S()) != hipSuccess ||
         hipMemcpyAsync(d_st, h_st, count * sizeof(State),
                        hipMemcpyHostToDevice, S()) != hipSuccess ||
-        hipMemcpyAsync(d_slots, h_slots, count * sizeof(uint32_t),
-                       hipMemcpyHostToDevice, S()) != hipSuccess ||
         hipMemsetAsync(d_count, 0, sizeof(uint32_t), S()) != hipSuccess ||
         hipMemsetAsync(d_bytes, 0, count * sizeof(uint64_t), S()) !=
             hipSuccess ||
@@ -837,28 +1034,23 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
         hipStreamSynchronize(S()) != hipSuccess)
       rc = -1;
   }
-  // Waves per LP workgroup: 4 (measured on 4,096 C3 LPs: 334 solves/s
-  // with 4, 318 with 8; every value gives the same results for a given
-  // workgroup size, and the size is fixed so that an LP's result never
-  // depends on the batch it is in).  ACD_BATCH_WAVES=2|8 overrides it.
-  int wpb = 4;
-  const char* fixed = getenv("ACD_BATCH_WAVES");
-  if (fixed && (atoi(fixed) == 2 || atoi(fixed) == 8)) wpb = atoi(fixed);
+  const BatchBufs bufs = {d_args, d_st, d_count, d_done, d_bytes, d_flags};
+  Group grp[kWpbChoices] = {};
+  const bool graphs = use_graphs();
   // Relaunch over the unfinished LPs (compacted between rounds) until all
-  // are done; the budget per launch is 32 inner iterations.
+  // are done; the budget per launch is 32 inner iterations, 8 launches per
+  // group per round (one captured graph per group while its list holds).
   uint32_t active = (uint32_t)count;
   uint32_t* live = h_slots;
   uint32_t done = 0;
+  if (!rc && upload_groups(live, active, choice, h_tmp, d_slots, grp))
+    rc = -1;
   if (!rc) {
     hipEventRecord(e0, S());
     k_stamp<<<1, 1, 0, S()>>>(d_t0);
     for (int round = 0; !rc && active; round++) {
-      for (int l = 0; l < 8; l++) {
-        launch_batch(key, wpb, active, d_args, d_st, d_slots, 32, d_count,
-                     d_done, d_bytes, d_flags);
-        launches++;
-      }
-      if (hipGetLastError() != hipSuccess ||
+      const double t = now_s();
+      if (run_round(key, grp, kRoundLaunches, kBatchBudget, bufs, graphs) ||
           hipMemcpyAsync(&done, d_count, sizeof done, hipMemcpyDeviceToHost,
                          S()) != hipSuccess ||
           hipStreamSynchronize(S()) != hipSuccess) {
@@ -866,9 +1058,48 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
         rc = -1;
         break;
       }
+      for (int i = 0; i < kWpbChoices; i++)
+        if (grp[i].n) launches += kRoundLaunches;
       if (done >= count) break;
-      // Every 16 rounds, drop finished LPs from the launch list.
-      if (round % 16 == 15) {
+      // The tuner: each candidate configuration runs one round (all of
+      // them advance every unfinished LP by the same iteration budget, so
+      // the round times compare), the fastest is kept; again once half of
+      // the LPs it was tuned on have finished.
+      bool regroup = false;
+      if (trial >= 0) {
+        trial_s[trial] = now_s() - t;
+        if (++trial < kConfigs) {
+          cfg = trial;
+        } else {
+          // Another configuration replaces 4 waves everywhere only when
+          // its round is clearly shorter: on 4,096 mixed-size LPs the
+          // candidates come within a few percent of each other and the
+          // ones that slow the small LPs down delay their completion.
+          cfg = 0;
+          for (int c = 1; c < kConfigs; c++)
+            if (trial_s[c] < kTuneMargin * trial_s[cfg]) cfg = c;
+          if (tune_log)
+            fprintf(stderr, "batch tune at %u unfinished: %.2f %.2f %.2f "
+                    "%.2f %.2f ms -> %d\n", (unsigned)(count - done),
+                    1e3 * trial_s[0], 1e3 * trial_s[1], 1e3 * trial_s[2],
+                    1e3 * trial_s[3], 1e3 * trial_s[4], cfg);
+          if (first_cfg < 0) {
+            first_cfg = cfg;
+            g_config = cfg;
+          }
+          trial = -1;
+          tuned_at = (uint32_t)(count - done);
+        }
+        regroup = true;
+      } else if (tune && 2 * (count - done) <= tuned_at &&
+                 count - done >= kTuneMinLps) {
+        trial = 0;
+        cfg = 0;
+        regroup = true;
+      }
+      // Every 16 rounds, and when the configuration changes, drop
+      // finished LPs from the launch lists.
+      if (round % 16 == 15 || regroup) {
         if (hipMemcpy(h_st, d_st, count * sizeof(State),
                       hipMemcpyDeviceToHost) != hipSuccess) {
           rc = -1;
@@ -878,17 +1109,19 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
         for (uint32_t i = 0; i < active; i++) {
           const uint32_t lp = live[i];
           if (h_st[lp].pc != PC_DONE && !h_st[lp].error) live[k++] = lp;
+          choice[lp] = (uint8_t)lp_choice(cls[lp], forced, cfg);
         }
         active = k;
-        if (active &&
-            hipMemcpyAsync(d_slots, live, active * sizeof(uint32_t),
-                           hipMemcpyHostToDevice, S()) != hipSuccess)
+        if (upload_groups(live, active, choice, h_tmp, d_slots, grp)) {
           rc = -1;
+          break;
+        }
       }
     }
     hipEventRecord(e1, S());
     hipEventSynchronize(e1);
   }
+  drop_graphs(grp);
   float ms = 0;
   if (!rc) hipEventElapsedTime(&ms, e0, e1);
   uint64_t t0 = 0;
@@ -926,7 +1159,11 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
       total_bytes += h_bytes[i];
     }
     if (stats) {
-      stats->waves_per_lp = wpb;
+      // The configuration of the first tuning (the bulk of the batch).
+      const int c0 = first_cfg >= 0 ? first_cfg : cfg;
+      for (int c = 0; c < kClasses; c++) stats->waves_per_lp[c] = 0;
+      for (size_t i = 0; i < count; i++)
+        stats->waves_per_lp[cls[i]] = kWpbOf[lp_choice(cls[i], forced, c0)];
       stats->device_ms = ms;
       stats->launches = launches;
       stats->inner_iterations = inner;
@@ -946,12 +1183,17 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   free(h_args);
   free(h_st);
   free(h_slots);
+  free(h_tmp);
   free(h_done);
   free(h_bytes);
+  free(cls);
+  free(choice);
   return rc;
 }
 
 // ---- slot pool for continuous batching ----------------------------------
+// Each round splits the active slots by workgroup size (the tuned choice of
+// the slot's size class) and runs the groups on their own streams.
 struct acd_dev_pool {
   uint32_t cap;
   int key;
@@ -960,6 +1202,8 @@ struct acd_dev_pool {
   uint32_t *d_slots, *d_count, *d_flags;
   uint64_t *d_done, *d_bytes;
   uint32_t* h_flags;  // pinned
+  uint32_t* h_tmp;
+  uint8_t* choice;    // per slot
 };
 
 struct acd_dev_pool* acd_dev_pool_create(uint32_t cap, int precision) {
@@ -972,7 +1216,10 @@ struct acd_dev_pool* acd_dev_pool_create(uint32_t cap, int precision) {
   // be exact there).
   p->key = precision == ACD_DEV_MIXED ? K_MIX_F32
            : precision == ACD_DEV_FP32 ? K_F32 : K_F64;
-  if (hipMalloc((void**)&p->d_args, cap * sizeof(Args)) != hipSuccess ||
+  p->h_tmp = (uint32_t*)malloc(cap * sizeof(uint32_t));
+  p->choice = (uint8_t*)calloc(cap, 1);
+  if (!p->h_tmp || !p->choice ||
+      hipMalloc((void**)&p->d_args, cap * sizeof(Args)) != hipSuccess ||
       hipMalloc((void**)&p->d_st, cap * sizeof(State)) != hipSuccess ||
       hipMalloc((void**)&p->d_slots, cap * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc((void**)&p->d_count, sizeof(uint32_t)) != hipSuccess ||
@@ -1003,6 +1250,8 @@ void acd_dev_pool_free(struct acd_dev_pool* p) {
   hipFree(p->d_done);
   hipFree(p->d_bytes);
   if (p->h_flags) hipHostFree(p->h_flags);
+  free(p->h_tmp);
+  free(p->choice);
   free(p);
 }
 
@@ -1017,6 +1266,9 @@ int acd_dev_pool_admit(struct acd_dev_pool* p, uint32_t slot,
         acd_dev_sell_make_fp32(&om->a) || acd_dev_sell_make_fp32(&om->at))
       return -1;
   }
+  const int forced = forced_choice();
+  p->choice[slot] =
+      (uint8_t)lp_choice(size_class(lp), forced, g_config);
   Args a = make_args(lp, p->key);
   a.ev = nullptr;
   a.ev_cap = 0;
@@ -1045,16 +1297,19 @@ int acd_dev_pool_round(struct acd_dev_pool* p, const uint32_t* slots,
                        uint32_t n, uint32_t launches, uint64_t budget,
                        uint32_t** flags) {
   if (n > p->cap) return -1;
+  for (uint32_t i = 0; i < n; i++)
+    if (slots[i] >= p->cap) return -1;
   if (n) {
-    if (hipMemcpyAsync(p->d_slots, slots, n * sizeof(uint32_t),
-                       hipMemcpyHostToDevice, S()) != hipSuccess)
+    Group grp[kWpbChoices] = {};
+    const BatchBufs bufs = {p->d_args,  p->d_st,   p->d_count,
+                            p->d_done,  p->d_bytes, p->d_flags};
+    if (upload_groups(slots, n, p->choice, p->h_tmp, p->d_slots, grp) ||
+        run_round(p->key, grp, launches, budget, bufs, false)) {
+      acd_dev_record_error(-1, "pool round");
       return -1;
-    for (uint32_t l = 0; l < launches; l++)
-      launch_batch(p->key, 4, n, p->d_args, p->d_st, p->d_slots, budget,
-                   p->d_count, p->d_done, p->d_bytes, p->d_flags);
+    }
   }
-  if (hipGetLastError() != hipSuccess ||
-      hipMemcpyAsync(p->h_flags, p->d_flags, p->cap * sizeof(uint32_t),
+  if (hipMemcpyAsync(p->h_flags, p->d_flags, p->cap * sizeof(uint32_t),
                      hipMemcpyDeviceToHost, S()) != hipSuccess ||
       hipStreamSynchronize(S()) != hipSuccess) {
     acd_dev_record_error(-1, "pool round");

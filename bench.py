@@ -210,6 +210,7 @@ def batch_leg(lib, count, prec, stop_kkt, steps=1, fp64_sample=8,
         done_all.append(done)
         inner += int(info.inner_iterations)
         alg += info.alg_bytes
+        wpb = list(info.waves_per_lp)
         bad = [i for i, r in enumerate(out) if r[0] != 0]
         if bad:
             raise RuntimeError("batch: %d LPs did not converge" % len(bad))
@@ -242,6 +243,7 @@ def batch_leg(lib, count, prec, stop_kkt, steps=1, fp64_sample=8,
         "p50_ms": float(np.percentile(done, 50)),
         "p99_ms": float(np.percentile(done, 99)),
         "kkt_max": float(max(kkts)), "inner_iterations": inner // steps,
+        "waves_per_lp_by_class": wpb,
         "roofline": {"kernel": "k_batch", "achieved": round(gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4)},

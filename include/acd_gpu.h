@@ -114,13 +114,22 @@ enum acd_solve_error {
         ACD_SOLVE_UNBOUNDED = 1,   /* an iterate is not finite */
         ACD_SOLVE_ESTIMATE = 2     /* a long step's model estimate > 0 */
 };
+#define ACD_BATCH_CLASSES 8
 struct acd_batch_info {
         double device_ms;            /* device time of the whole batch */
         uint64_t launches;           /* batch kernel launches */
         uint64_t inner_iterations;   /* summed over the LPs */
         double alg_bytes;            /* algorithmic bytes moved, summed */
-        int waves_per_lp;            /* wavefronts per LP workgroup, picked
-                                        by a timed trial on a sample */
+        /* Wavefronts per LP workgroup (1, 2 or 4) used for each size
+         * class: class k holds the LPs whose wider operator (A or A') has
+         * at most 2^k slices of 64 rows, the last class the rest; 0 where
+         * the batch had no LP of the class.  Batches of 192 LPs or more
+         * tune it on their own rounds (each candidate assignment runs one
+         * round, the fastest is kept; again when half of the LPs have
+         * finished) and report the first choice; smaller batches reuse
+         * the last one; ACD_BATCH_WAVES=1|2|4 forces one size.  Results do
+         * not depend on it. */
+        int waves_per_lp[ACD_BATCH_CLASSES];
 };
 /* alm_solve_ex on `count` independent ALM objects at once: one workgroup
  * per LP, every LP on the device together, repeated launches of 32 inner

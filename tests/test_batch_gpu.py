@@ -139,3 +139,36 @@ def test_single_small_lp_routes_to_the_workgroup_solver(gpu):
     gpu.free_matrix(m)
     alone, _, _ = batch(gpu, [lp], 0)
     assert same(r_auto[:4], r_wg[:4]) and same(r_auto[:4], alone[0])
+
+
+def test_result_does_not_depend_on_the_workgroup_size(gpu, monkeypatch):
+    """Every op is reduced as 4 virtual waves, so workgroups of 1, 2 or 4
+    wavefronts give bit-identical results."""
+    lps = I.batch(10, seed=13, mmin=20, mmax=600)
+    outs = []
+    for w in (1, 2, 4):
+        monkeypatch.setenv("ACD_BATCH_WAVES", str(w))
+        print(f"[batch] waves={w}", flush=True)
+        out, _, info = batch(gpu, lps, 1)
+        assert set(info.waves_per_lp) <= {0, w}
+        outs.append(out)
+    for o in outs[1:]:
+        for i, (r0, r) in enumerate(zip(outs[0], o)):
+            assert same(r0, r), i
+
+
+def test_tuned_batch_gives_the_forced_results(gpu, monkeypatch):
+    """A batch large enough to tune switches workgroup sizes between rounds
+    (one trial round per candidate); its results are those of a fixed
+    size."""
+    lps = I.batch(200, seed=14, mmin=20, mmax=200)
+    monkeypatch.delenv("ACD_BATCH_WAVES", raising=False)
+    print("[batch] tuned 200", flush=True)
+    tuned, _, info = batch(gpu, lps, 1)
+    assert any(v in (1, 2, 4) for v in info.waves_per_lp)
+    assert all(v in (0, 1, 2, 4) for v in info.waves_per_lp)
+    monkeypatch.setenv("ACD_BATCH_WAVES", "4")
+    sample = [0, 7, 50, 123, 199]
+    fixed, _, _ = batch(gpu, [lps[i] for i in sample], 1)
+    for k, i in enumerate(sample):
+        assert same(tuned[i], fixed[k]), i

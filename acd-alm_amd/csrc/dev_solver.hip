@@ -95,7 +95,15 @@ __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int kPer = kMaxBlocks / 64;
   double v0[kPer], v1[kPer];
+  // The per-op counters (thread t holds word t of the four arrays) and
+  // the phase timers (thread 0) are read in the same round as the
+  // partials, so updating them costs no extra memory round trip.
+  uint64_t* pw = reinterpret_cast<uint64_t*>(a.prof);
+  uint64_t pf = 0, cp[4] = {0, 0, 0, 0};
   {
+    if (threadIdx.x < 128) pf = pw[threadIdx.x];
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 4; k++) cp[k] = a.prof->ctl_phase[k];
     const double* p0 = a.partials + (size_t)(2 * w) * nblocks;
     const double* p1 = p0 + nblocks;
 #pragma unroll
@@ -115,16 +123,17 @@ __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
   const int32_t op = ls.op;
   const bool ran = op != OP_NONE;
   const bool live = ls.pc != PC_DONE && !ls.error;
-  // Per-op counters: lanes 0..3 each update one, in parallel.
-  if (threadIdx.x < 4 && live && ran && op >= 0 && op < 32) {
+  // Per-op counters: the four threads holding this op's words.
+  if (threadIdx.x < 128 && (threadIdx.x & 31) == (uint32_t)op && live &&
+      ran && op >= 0 && op < 32) {
     uint64_t d;
-    switch (threadIdx.x) {
+    switch (threadIdx.x >> 5) {
       case 0: d = 1; break;
       case 1: d = t_entry - ls.t_last; break;
       case 2: d = t_entry - ls.t_step; break;
       default: d = (uint64_t)alg_bytes(op, ls.op_flags, a); break;
     }
-    reinterpret_cast<uint64_t*>(a.prof)[threadIdx.x * 32 + op] += d;
+    pw[threadIdx.x] = pf + d;
   }
 #pragma unroll
   for (int kk = 0; kk < 2; kk++) {
@@ -148,10 +157,10 @@ __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
     c.advance();
     ls.t_last = wall_clock64();
     if (ran) {
-      a.prof->ctl_phase[0] += t_loaded - t_entry;
-      a.prof->ctl_phase[1] += t_comb - t_loaded;
-      a.prof->ctl_phase[2] += ls.t_last - t_comb;
-      a.prof->ctl_phase[3] += 1;
+      a.prof->ctl_phase[0] = cp[0] + (t_loaded - t_entry);
+      a.prof->ctl_phase[1] = cp[1] + (t_comb - t_loaded);
+      a.prof->ctl_phase[2] = cp[2] + (ls.t_last - t_comb);
+      a.prof->ctl_phase[3] = cp[3] + 1;
     }
   }
   __syncthreads();

@@ -268,14 +268,22 @@ struct Ops {
   __device__ __forceinline__ void spmv_rows_pt(const Sell& M, const X* x0,
                                                const X* x1, PreF&& pre,
                                                E&& epi) {
-    uint32_t sl = g.gwave;
+    // Round k gives wave w slice k*nwaves + (w + k) mod nwaves.  Slices
+    // in a window of 256 rows are ordered by row length (the reference's
+    // permutation), so a plain stride of nwaves (a multiple of 4) would
+    // give every wave the same position in each window -- some waves all
+    // the longest slices; the rotation spreads them.
+    uint32_t rk = g.gwave, bk = 0;
+    uint32_t sl = rk;
     if (sl >= M.nslices) return;
     uint64_t base = M.off[sl], end = M.off[sl + 1];
     for (;;) {
       // The next slice's bounds and each next chunk of 4 columns/values
       // are loaded while the current chunk's gathers are in flight; the
       // accumulation order is unchanged.
-      const uint32_t nsl = sl + g.nwaves;
+      bk += g.nwaves;
+      rk = rk + 1 == g.nwaves ? 0u : rk + 1;
+      const uint32_t nsl = bk + rk;
       uint64_t nbase = 0, nend = 0;
       if (nsl < M.nslices) {
         nbase = M.off[nsl];

@@ -487,6 +487,19 @@ def main():
             "share": round(step_us[i] / step_us.sum(), 3)}
             for i in top if step_us[i] > 0},
     }
+    # The stand-alone SpMV on the other headline matrix: the set-cover A is
+    # bound by random gathers, the multicommodity operators stream.
+    spmv_other = None
+    if args.workload == "setcover" and not args.no_batch:
+        try:
+            lp2 = instances.multicommodity()
+            mat2 = lib.make_matrix(lp2, 0)
+            spmv_other = dict(spmv_leg(lib, mat2, lp2),
+                              workload="multicommodity flow LP, m=%d n=%d "
+                              "nnz=%d" % (lp2.nrows, lp2.ncols, lp2.nnz))
+            lib.free_matrix(mat2)
+        except Exception as e:  # reported, never silently replaced
+            spmv_other = {"error": str(e)[-300:]}
     batch = None
     if not args.no_batch:
         try:
@@ -525,6 +538,7 @@ def main():
         "solver_device_ms": device_ms / args.steps,
         "roofline": roofline,
         "spmv": spmv,
+        "spmv_mcf": spmv_other,
         "batch": batch,
         "cpu_baseline": cpu,
         "setup_s": setup_s,

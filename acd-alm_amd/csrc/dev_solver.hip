@@ -811,7 +811,7 @@ int upload_groups(const uint32_t* live, uint32_t n, const uint8_t* choice,
 // again whenever the number of unfinished LPs has halved, down to
 // kTuneMinLps; smaller batches use the last tuned configuration.
 constexpr size_t kTuneMinLps = 192;
-constexpr double kTuneMargin = 0.95;
+constexpr double kTuneMargin = 0.90;
 constexpr uint32_t kRoundLaunches = 8;
 constexpr uint64_t kBatchBudget = 32;
 
@@ -1081,9 +1081,10 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
           cfg = trial;
         } else {
           // Another configuration replaces 4 waves everywhere only when
-          // its round is clearly shorter: on 4,096 mixed-size LPs the
-          // candidates come within a few percent of each other and the
-          // ones that slow the small LPs down delay their completion.
+          // its round is at least 10% shorter: on 4,096 mixed-size LPs
+          // the candidates' rounds differ by a few percent (noise can
+          // reach 5%), and the ones that slow the small LPs down delay
+          // their completion without speeding up the large ones.
           cfg = 0;
           for (int c = 1; c < kConfigs; c++)
             if (trial_s[c] < kTuneMargin * trial_s[cfg]) cfg = c;

@@ -181,16 +181,17 @@ def batch_cpu_baseline(lps, sample=32, procs=16):
 
 
 def batch_leg(lib, count, prec, stop_kkt, steps=1, fp64_sample=8,
-              cpu=True):
-    """Batched solve of `count` independent LPs (m log-uniform in [50, 2000],
-    n = 2-3 m, 5 nonzeros per column, random +-U[0.5, 2] values; seeded),
+              cpu=True, mmin=50, mmax=2000):
+    """Batched solve of `count` independent LPs (m log-uniform in [mmin,
+    mmax], default [50, 2000], n = 2-3 m, 5 nonzeros per column, random
+    +-U[0.5, 2] values; seeded),
     all device-resident, to the relative KKT target.  Also times the plain
     FP64 single-LP form (alm_solve_ex, one LP at a time) on the first
     `fp64_sample` LPs for the ratio."""
     from acdalm import instances, kkt
     L = lib.lib
     t0 = time.perf_counter()
-    lps = instances.batch(count, seed=4)
+    lps = instances.batch(count, seed=4, mmin=mmin, mmax=mmax)
     mats = [lib.make_matrix(lp, 0) for lp in lps]
     alms = [lib.alm_make(m, lp) for m, lp in zip(mats, lps)]
     setup = time.perf_counter() - t0
@@ -236,8 +237,8 @@ def batch_leg(lib, count, prec, stop_kkt, steps=1, fp64_sample=8,
     rate = count * steps / (dev_ms * 1e-3)
     gbs = alg / (dev_ms * 1e-3) / 1e9
     return {
-        "workload": "%d independent LPs, m log-uniform [50,2000], n=2-3m, "
-                    "5 nnz/column" % count,
+        "workload": "%d independent LPs, m log-uniform [%d,%d], n=2-3m, "
+                    "5 nnz/column" % (count, mmin, mmax),
         "precision": "mixed" if prec == 1 else "fp64",
         "solves_per_s": rate, "device_ms_per_batch": dev_ms / steps,
         "p50_ms": float(np.percentile(done, 50)),
@@ -351,6 +352,10 @@ def main():
     ap.add_argument("--seconds", type=float, default=1.0,
                     help="serve: length of the arrival window")
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-mmin", type=int, default=50,
+                    help="batched LPs: smallest row count (log-uniform)")
+    ap.add_argument("--batch-mmax", type=int, default=2000,
+                    help="batched LPs: largest row count")
     ap.add_argument("--no-batch", action="store_true",
                     help="setcover workload: skip the batched-solve leg")
     ap.add_argument("--copy-check", type=int, default=0, metavar="BYTES",
@@ -394,7 +399,8 @@ def main():
         L.acd_synchronize()
         t = time.perf_counter()
         b = batch_leg(lib, args.batch_count, prec, args.stop_kkt,
-                      steps=args.steps, cpu=not args.no_cpu)
+                      steps=args.steps, cpu=not args.no_cpu,
+                      mmin=args.batch_mmin, mmax=args.batch_mmax)
         total = time.perf_counter() - t
         return ({
             "metric": METRIC, "value": b["solves_per_s"], "unit": "solves/s",

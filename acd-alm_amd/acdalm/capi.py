@@ -48,6 +48,21 @@ class BatchInfo(C.Structure):
                 ("waves_per_lp", C.c_int * 8)]
 
 
+class MpsLP(C.Structure):
+    """struct acd_mps_lp (include/acd_gpu.h)."""
+    _fields_ = [("nrows", C.c_size_t), ("ncols", C.c_size_t),
+                ("nnz", C.c_size_t), ("nstructural", C.c_size_t),
+                ("rows", C.POINTER(C.c_uint32)),
+                ("cols", C.POINTER(C.c_uint32)),
+                ("values", C.POINTER(C.c_double)),
+                ("rhs", C.POINTER(C.c_double)),
+                ("linear", C.POINTER(C.c_double)),
+                ("lower", C.POINTER(C.c_double)),
+                ("upper", C.POINTER(C.c_double)),
+                ("objective_offset", C.c_double), ("maximize", C.c_int),
+                ("error", C.c_char * 128)]
+
+
 class SolveStats(C.Structure):
     """struct acd_solve_stats (include/acd_gpu.h)."""
     _fields_ = [("inner_iterations", C.c_uint64),
@@ -103,7 +118,10 @@ class LogCapture:
 class CLib:
     """Bindings for a library exporting the original interface."""
 
-    def __init__(self, path: str = LIB_PATH, *, gpu_extras: bool = None):
+    def __init__(self, path: str = LIB_PATH, *, gpu_extras: bool = None,
+                 init: bool = True):
+        """init=False binds the functions without sparse_matrix_init (no
+        device needed: host-only entry points such as the MPS parser)."""
         mode = getattr(os, "RTLD_LOCAL", 0) | getattr(os, "RTLD_NOW", 2)
         self.path = path
         self.lib = C.CDLL(path, mode=mode)
@@ -174,7 +192,8 @@ class CLib:
         self.gpu = gpu_extras
         if gpu_extras:
             self._bind_gpu()
-        L.sparse_matrix_init()
+        if init:
+            L.sparse_matrix_init()
 
     def _bind_gpu(self):
         L = self.lib
@@ -218,6 +237,9 @@ class CLib:
             ("acd_last_solve_bytes", None, [C.POINTER(C.c_double),
                                             C.POINTER(C.c_double), C.c_int]),
             ("acd_pass_name", C.c_char_p, [C.c_int]),
+            ("acd_mps_parse", C.c_int, [vp, C.POINTER(MpsLP)]),
+            ("acd_mps_free", None, [C.POINTER(MpsLP)]),
+            ("acd_alm_read_mps", vp, [vp, C.POINTER(MpsLP)]),
         ]:
             fn = getattr(L, name, None)
             if fn is None:
@@ -421,6 +443,51 @@ class CLib:
                 out[self.lib.acd_pass_name(i).decode()] = (
                     int(cnt[i]), float(sus[i]), float(by[i]))
         return out
+
+    # ---- MPS ----
+    def _fopen(self, path):
+        libc = C.CDLL(None)
+        libc.fopen.restype = vp
+        libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+        libc.fclose.argtypes = [vp]
+        f = libc.fopen(path.encode(), b"r")
+        if not f:
+            raise OSError(f"cannot open {path}")
+        return libc, f
+
+    def mps_parse(self, path):
+        """acd_mps_parse: dict of numpy arrays (copies), or raises
+        ValueError with the parser's message."""
+        libc, f = self._fopen(path)
+        lp = MpsLP()
+        rc = self.lib.acd_mps_parse(f, C.byref(lp))
+        libc.fclose(f)
+        if rc:
+            raise ValueError(lp.error.decode())
+        arr = lambda p, n: np.ctypeslib.as_array(p, (n,)).copy() if n else \
+            np.zeros(0, np.dtype(p._type_))
+        out = {"nrows": lp.nrows, "ncols": lp.ncols,
+               "nstructural": lp.nstructural,
+               "objective_offset": lp.objective_offset,
+               "maximize": bool(lp.maximize),
+               "rows": arr(lp.rows, lp.nnz), "cols": arr(lp.cols, lp.nnz),
+               "values": arr(lp.values, lp.nnz),
+               "rhs": arr(lp.rhs, lp.nrows),
+               "linear": arr(lp.linear, lp.ncols),
+               "lower": arr(lp.lower, lp.ncols),
+               "upper": arr(lp.upper, lp.ncols)}
+        self.lib.acd_mps_free(C.byref(lp))
+        return out
+
+    def alm_read_mps(self, path):
+        """acd_alm_read_mps: (alm, info) or raises ValueError."""
+        libc, f = self._fopen(path)
+        info = MpsLP()
+        a = self.lib.acd_alm_read_mps(f, C.byref(info))
+        libc.fclose(f)
+        if not a:
+            raise ValueError(info.error.decode())
+        return a, info
 
     def stats(self):
         st = SolveStats()

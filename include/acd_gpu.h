@@ -148,6 +148,36 @@ int alm_solve_batch(size_t count, alm_t * const * alms, size_t niter,
                     double * OUT_done_ms, uint64_t * OUT_inner,
                     struct acd_batch_info * OUT_info);
 
+/* ---- MPS input ---- */
+/* An LP read from (free or blank-free fixed) MPS: sections NAME, ROWS,
+ * COLUMNS (integrality markers ignored), RHS, RANGES, BOUNDS, OBJSENSE,
+ * ENDATA.  It is put in the ALM form min c'x s.t. Ax = b, l <= x <= u:
+ * each L, G or ranged row whose interval [lo, hi] is not a single point
+ * gets a slack column, after the structural columns: a'x - s = lo with
+ * 0 <= s <= hi - lo when lo is finite, else a'x + s = hi with s >= 0.
+ * OBJSENSE MAX negates the costs.  Arrays are malloc'ed; the COO holds
+ * the COLUMNS entries in input order, then one entry per slack. */
+struct acd_mps_lp {
+        size_t nrows, ncols, nnz;       /* ncols includes the slacks */
+        size_t nstructural;             /* columns named in COLUMNS */
+        uint32_t * rows, * cols;
+        double * values;
+        double * rhs, * linear, * lower, * upper;
+        double objective_offset;        /* constant term of c'x (minus
+                                           the objective row's RHS) */
+        int maximize;                   /* OBJSENSE MAX: costs negated */
+        char error[128];                /* why parsing failed */
+};
+/* 0, or -1 with OUT_lp->error set (nothing to free then). */
+int acd_mps_parse(FILE * stream, struct acd_mps_lp * OUT_lp);
+void acd_mps_free(struct acd_mps_lp * lp);
+/* acd_mps_parse, sparse_matrix_make (unpermuted) and alm_make with free
+ * multipliers; like alm_read, the ALM object references the matrix
+ * (release it with sparse_matrix_free(alm_matrix(alm)) taken before
+ * alm_free).  OUT_info (optional) receives the sizes, offset and error
+ * text; its arrays are already freed.  NULL on error. */
+alm_t * acd_alm_read_mps(FILE * stream, struct acd_mps_lp * OUT_info);
+
 /* ---- request loop (continuous batching) ---- */
 typedef struct acd_server acd_server_t;
 /* A request loop on the current device.  hbm_budget bounds the summed

@@ -107,6 +107,31 @@ def spmv_leg(lib, mat, lp, reps=50):
     return out
 
 
+def cpu_spmv(lp, threads, repeat=10):
+    """The reference's sparse_matrix_multiply (A and A', FP64, its thread
+    pool with `threads` workers) on the same matrix, from the build in
+    oracle/_ref; mean seconds per product over `repeat` products."""
+    ref = os.path.join(HERE, "oracle", "_ref", "ref_bench")
+    if not os.path.exists(ref):
+        return {"error": "reference build absent"}
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "lp.lpb")
+        lp.write_binary(path)
+        r = subprocess.run([ref, path, str(threads), "spmv", str(repeat)],
+                           capture_output=True, text=True, timeout=900)
+    if r.returncode:
+        return {"error": r.stderr[-300:]}
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    out = {"threads": threads, "repeat": repeat, "kind": "reference"}
+    for key, tr in (("A", 0), ("AT", 1)):
+        ny, nx = (lp.ncols, lp.nrows) if tr else (lp.nrows, lp.ncols)
+        alg = lp.nnz * (8 + 4) + 8 * ((ny + 63) // 64 + 1) + 8 * nx + 8 * ny
+        sec = res["s_per_at" if tr else "s_per_a"]
+        out[key] = {"us": round(sec * 1e6, 1),
+                    "GB/s": round(alg / sec / 1e9, 2)}
+    return out
+
+
 def cpu_baseline(lp, inner_iterations, iters, threads):
     """Bounded CPU sample: `iters` APPROX iterations of the first ALM
     subproblem on the reference build (or the oracle restatement), scaled to
@@ -123,6 +148,12 @@ def cpu_baseline(lp, inner_iterations, iters, threads):
             res = json.loads(r.stdout.strip().splitlines()[-1])
             spi = res["s_per_iteration"]
             kind, cores = "reference", threads
+            # the same on one thread (a tenth of the iterations)
+            r1 = subprocess.run([ref, path, "1", "approx",
+                                 str(max(10, iters // 10))],
+                                capture_output=True, text=True, timeout=900)
+            spi1 = (json.loads(r1.stdout.strip().splitlines()[-1])
+                    ["s_per_iteration"] if r1.returncode == 0 else None)
         else:
             from pyoracle import Oracle
             ora = Oracle()
@@ -135,9 +166,12 @@ def cpu_baseline(lp, inner_iterations, iters, threads):
             ora.approx_free(a)
             ora.free_matrix(m)
             kind, cores = "port", 1
+            spi1 = spi
     per_solve = spi * inner_iterations
     return {"value": 1.0 / per_solve, "unit": "solves/s", "cores": cores,
             "kind": kind,
+            "value_1_thread": (1.0 / (spi1 * inner_iterations)
+                               if spi1 else None),
             "sample": ("%d APPROX iterations of the first ALM subproblem of "
                        "the same instance (%.4g s/iteration on %d thread%s), "
                        "scaled by the %d inner iterations of one solve"
@@ -476,6 +510,9 @@ def main():
     kkt_ok = bool(rc == 0 and res["kkt"] <= KKT_TOL)
     lat_ms = np.array(lat) * 1e3
     spmv = spmv_leg(lib, mat, lp)
+    cpu_threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    if not args.no_cpu:
+        spmv["cpu_reference"] = cpu_spmv(lp, cpu_threads)
 
     achieved = bytes_.sum() / (step_us.sum() * 1e-6) / 1e9
     names = [L.acd_pass_name(j).decode() for j in range(32)]
@@ -504,6 +541,8 @@ def main():
                               workload="multicommodity flow LP, m=%d n=%d "
                               "nnz=%d" % (lp2.nrows, lp2.ncols, lp2.nnz))
             lib.free_matrix(mat2)
+            if not args.no_cpu:
+                spmv_other["cpu_reference"] = cpu_spmv(lp2, cpu_threads)
         except Exception as e:  # reported, never silently replaced
             spmv_other = {"error": str(e)[-300:]}
     batch = None

@@ -446,7 +446,6 @@ struct Ops {
         VI* Zp = VN(NZP);
         const VI* Z = VN(NZ);
         VI* G1 = VN(NG);
-        VI* G2 = VN(NG2);
         VI* X = VN(NX);
         VI* PV = VN(NPV);
         const VC* iv = P(a.p_iv);
@@ -476,8 +475,9 @@ struct Ops {
             if (pcopy) PV[j] = xj;
             if (lazy) X[j] = (VI)(ls * (double)xj + lt * zj);
           }
+          // g2 = A'viol(y) + c is not stored: the rare re-step (FIXUP)
+          // recomputes it bit for bit from the same viol(y).
           const double g2 = gy + cj;
-          G2[j] = (VI)g2;
           const double stp = itheta * q.iv;
           const VI tv = (VI)minpd(q.u, maxpd(q.l, zj - g2 * stp));
           const double t = (double)tv;
@@ -550,11 +550,12 @@ struct Ops {
         VI* Zp = VN(NZP);
         const VI* Z = VN(NZ);
         VI* G1 = VN(NG);
-        const VI* G2 = VN(NG2);
         const VC* iv = P(a.p_iv);
         const bool needg = fl & F_NEEDG, restep = fl & F_RESTEP;
         const double itheta = s.op_a;
-        auto body = [&](uint32_t j, double gz) {
+        // gy = A'viol(y): viol(y) is still PASS1's (PASS2 wrote only the
+        // shadow copies), so g2 = gy + c is PASS1's value bit for bit.
+        auto body = [&](uint32_t j, double gy, double gz) {
           const double zj = (double)Z[j];
           double gg;
           if (needg) {
@@ -565,8 +566,9 @@ struct Ops {
           }
           double t;
           if (restep) {
+            const double g2 = (double)(VI)(gy + (double)pc[j]);
             const double stp = itheta * (double)iv[j];
-            const VI tv = (VI)clamp_lu(j, zj - (double)G2[j] * stp);
+            const VI tv = (VI)clamp_lu(j, zj - g2 * stp);
             Zp[j] = tv;
             t = (double)tv;
             p[1] = maxpd(p[1], fabs(t));
@@ -575,11 +577,17 @@ struct Ops {
           }
           p[0] += gg * (t - zj);
         };
-        if (needg)
+        if (needg && restep)
+          spmv_rows<2>(a.AT, VM(MYV), VM(MZV),
+                       [&](uint32_t j, double gy, double gz) { body(j, gy, gz); });
+        else if (needg)
           spmv_rows<1>(a.AT, VM(MZV), nullptr,
-                       [&](uint32_t j, double gz, double) { body(j, gz); });
+                       [&](uint32_t j, double gz, double) { body(j, 0.0, gz); });
+        else if (restep)
+          spmv_rows<1>(a.AT, VM(MYV), nullptr,
+                       [&](uint32_t j, double gy, double) { body(j, gy, 0.0); });
         else
-          each(a.n, [&](uint32_t j) { body(j, 0.0); });
+          each(a.n, [&](uint32_t j) { body(j, 0.0, 0.0); });
         break;
       }
       case OP_RESTART_COPY: {  // approx/approx.c:452-453
@@ -842,7 +850,7 @@ __device__ inline double alg_bytes(int32_t op, uint32_t fl, const Args& a) {
     case OP_PASS1: {
       const bool two = fl & F_TWO;
       double b = spT + m * si * (two ? 2 : 1);
-      b += n * (4 * sc + 3 * si);  // c, iv, l, u; z, g2, zp
+      b += n * (4 * sc + 2 * si);  // c, iv, l, u; z, zp
       if (fl & F_LONG) b += n * sc;
       if (two) b += n * si;
       if (fl & (F_PCOPY | F_LAZY)) b += n * si;
@@ -853,9 +861,11 @@ __device__ inline double alg_bytes(int32_t op, uint32_t fl, const Args& a) {
     case OP_PASS2:
       return spA + n * si + m * (16 + 4 * si);
     case OP_FIXUP: {
+      const bool g = fl & F_NEEDG, r = fl & F_RESTEP;
       double b = n * si;  // z
-      b += (fl & F_NEEDG) ? spT + m * si + n * (sc + si) : n * si;
-      b += (fl & F_RESTEP) ? n * (si + 3 * sc + si) : n * si;
+      if (g || r) b += spT + m * si * ((g && r) ? 2 : 1) + n * sc;  // + c
+      b += g ? n * si : n * si;  // g written, or g read
+      b += r ? n * (3 * sc + si) : n * si;  // iv, l, u; zp written / read
       return b;
     }
     case OP_RESTART_COPY: return 2 * n * si + 3 * m * si;

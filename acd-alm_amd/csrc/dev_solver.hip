@@ -742,14 +742,24 @@ struct Group {
 
 int run_round(int key, Group* grp, uint32_t launches, uint64_t budget,
               const BatchBufs& b, bool graphs) {
-  GroupStreams* gs = group_streams();
-  if (!gs) return -1;
-  if (hipEventRecord(gs->fork, S()) != hipSuccess) return -1;
+  // One non-empty group runs on the library stream itself: the group
+  // streams are created only when sizes are mixed (extra streams share
+  // the device's few hardware queues with the request loop's large-LP
+  // streams, and a batch launch queued behind a large LP's graph waits
+  // for it).
+  int groups = 0;
+  for (int i = 0; i < kWpbChoices; i++) groups += grp[i].n != 0;
+  GroupStreams* gs = nullptr;
+  if (groups > 1) {
+    gs = group_streams();
+    if (!gs) return -1;
+    if (hipEventRecord(gs->fork, S()) != hipSuccess) return -1;
+  }
   for (int i = 0; i < kWpbChoices; i++) {
     Group& g = grp[i];
     if (!g.n) continue;
-    hipStream_t st = gs->s[i];
-    if (hipStreamWaitEvent(st, gs->fork, 0) != hipSuccess) return -1;
+    hipStream_t st = gs ? gs->s[i] : S();
+    if (gs && hipStreamWaitEvent(st, gs->fork, 0) != hipSuccess) return -1;
     if (graphs && !g.have_exec) {
       hipGraph_t graph;
       if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) !=
@@ -770,8 +780,8 @@ int run_round(int key, Group* grp, uint32_t launches, uint64_t budget,
       for (uint32_t l = 0; l < launches; l++)
         launch_batch(key, kWpbOf[i], g.n, g.d_slots, budget, b, st);
     }
-    if (hipEventRecord(gs->join[i], st) != hipSuccess ||
-        hipStreamWaitEvent(S(), gs->join[i], 0) != hipSuccess)
+    if (gs && (hipEventRecord(gs->join[i], st) != hipSuccess ||
+               hipStreamWaitEvent(S(), gs->join[i], 0) != hipSuccess))
       return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -741,16 +741,16 @@ struct Group {
 };
 
 int run_round(int key, Group* grp, uint32_t launches, uint64_t budget,
-              const BatchBufs& b, bool graphs) {
-  // One non-empty group runs on the library stream itself: the group
-  // streams are created only when sizes are mixed (extra streams share
-  // the device's few hardware queues with the request loop's large-LP
-  // streams, and a batch launch queued behind a large LP's graph waits
-  // for it).
+              const BatchBufs& b, bool graphs, bool concurrent) {
+  // One non-empty group (or concurrent = false) runs on the library
+  // stream itself: the group streams are created only when sizes are
+  // mixed in a batch (extra streams share the device's few hardware
+  // queues with the request loop's large-LP streams, and a batch launch
+  // queued behind a large LP's graph waits for it).
   int groups = 0;
   for (int i = 0; i < kWpbChoices; i++) groups += grp[i].n != 0;
   GroupStreams* gs = nullptr;
-  if (groups > 1) {
+  if (groups > 1 && concurrent) {
     gs = group_streams();
     if (!gs) return -1;
     if (hipEventRecord(gs->fork, S()) != hipSuccess) return -1;
@@ -1069,7 +1069,8 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
     k_stamp<<<1, 1, 0, S()>>>(d_t0);
     for (int round = 0; !rc && active; round++) {
       const double t = now_s();
-      if (run_round(key, grp, kRoundLaunches, kBatchBudget, bufs, graphs) ||
+      if (run_round(key, grp, kRoundLaunches, kBatchBudget, bufs, graphs,
+                    true) ||
           hipMemcpyAsync(&done, d_count, sizeof done, hipMemcpyDeviceToHost,
                          S()) != hipSuccess ||
           hipStreamSynchronize(S()) != hipSuccess) {
@@ -1213,7 +1214,8 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
 
 // ---- slot pool for continuous batching ----------------------------------
 // Each round splits the active slots by workgroup size (the tuned choice of
-// the slot's size class) and runs the groups on their own streams.
+// the slot's size class) and launches the groups one after the other on
+// the library stream.
 struct acd_dev_pool {
   uint32_t cap;
   int key;
@@ -1324,7 +1326,7 @@ int acd_dev_pool_round(struct acd_dev_pool* p, const uint32_t* slots,
     const BatchBufs bufs = {p->d_args,  p->d_st,   p->d_count,
                             p->d_done,  p->d_bytes, p->d_flags};
     if (upload_groups(slots, n, p->choice, p->h_tmp, p->d_slots, grp) ||
-        run_round(p->key, grp, launches, budget, bufs, false)) {
+        run_round(p->key, grp, launches, budget, bufs, false, false)) {
       acd_dev_record_error(-1, "pool round");
       return -1;
     }

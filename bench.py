@@ -214,6 +214,86 @@ def batch_cpu_baseline(lps, sample=32, procs=16):
                        "rule (%.1f s wall)" % (sample, procs, wall))}
 
 
+def serve_cpu_baseline(lps, large, arrivals, large_inner, procs=16,
+                       sample=32, large_iters=20):
+    """The reference's CPU path replayed on the same arrival trace: a
+    server with `procs` host threads where a small LP takes one thread
+    (pool = NULL) and a large LP all of them (its thread pool).  Service
+    times: each small LP's from a log-log interpolation over `sample`
+    small LPs solved by the reference (ALM, its stopping rule, one
+    process each, `procs` at a time); a large LP's from `large_iters`
+    APPROX iterations on `procs` threads scaled by `large_inner`, the
+    inner iterations the device solve of one large LP took.  FIFO
+    admission; p50/p99 of completion minus arrival."""
+    ref = os.path.join(HERE, "oracle", "_ref", "ref_bench")
+    if not os.path.exists(ref):
+        return {"value": None, "error": "reference build absent"}
+    small = [i for i in range(len(lps)) if not large[i]]
+    order = sorted(small, key=lambda i: lps[i].nnz)
+    pick = [order[(k * len(order)) // sample] for k in range(min(sample, len(order)))]
+    times = {}
+    with tempfile.TemporaryDirectory() as td:
+        jobs = []
+        for k, i in enumerate(pick):
+            path = os.path.join(td, "lp%d.lpb" % k)
+            lps[i].write_binary(path)
+            jobs.append((i, path))
+        running = []
+        for i, path in jobs:
+            while len(running) >= procs:
+                j, t0, pr = running.pop(0)
+                pr.wait()
+                times[j] = time.perf_counter() - t0
+            running.append((i, time.perf_counter(), subprocess.Popen(
+                [ref, path, "1", "alm"], stdout=subprocess.DEVNULL,
+                stderr=subprocess.DEVNULL)))
+        for j, t0, pr in running:
+            pr.wait()
+            times[j] = time.perf_counter() - t0
+        spi = None
+        big = [i for i in range(len(lps)) if large[i]]
+        if big:
+            path = os.path.join(td, "large.lpb")
+            lps[big[0]].write_binary(path)
+            r = subprocess.run([ref, path, str(procs), "approx",
+                                str(large_iters)], capture_output=True,
+                               text=True, timeout=900)
+            if r.returncode == 0:
+                spi = json.loads(r.stdout.strip().splitlines()[-1])[
+                    "s_per_iteration"]
+    xs = np.log([lps[i].nnz for i in sorted(times, key=lambda i: lps[i].nnz)])
+    ys = np.log([times[i] for i in sorted(times, key=lambda i: lps[i].nnz)])
+    service = np.array([
+        (spi * large_inner if spi else np.nan) if large[i]
+        else float(np.exp(np.interp(np.log(lps[i].nnz), xs, ys)))
+        for i in range(len(lps))])
+    # replay: `procs` workers, FIFO; a large LP waits for all of them
+    free_at = np.zeros(procs)
+    done = np.zeros(len(lps))
+    for i in np.argsort(arrivals, kind="stable"):
+        if large[i]:
+            start = max(arrivals[i], free_at.max())
+            free_at[:] = start + service[i]
+        else:
+            w = int(np.argmin(free_at))
+            start = max(arrivals[i], free_at[w])
+            free_at[w] = start + service[i]
+        done[i] = start + service[i]
+    lat = (done - np.asarray(arrivals)) * 1e3
+    return {"value": float(len(lps) / done.max()), "unit": "solves/s",
+            "cores": procs, "kind": "reference",
+            "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)),
+            "large_service_s": (spi * large_inner) if spi else None,
+            "sample": ("trace replay on %d host threads; small-LP service "
+                       "times from %d reference ALM solves (one thread "
+                       "each, median %.3g s), large-LP service time %.4g "
+                       "s/iteration x %d inner iterations on %d threads"
+                       % (procs, len(times), float(np.median(list(
+                           times.values()))), spi or float("nan"),
+                          large_inner, procs))}
+
+
 def batch_leg(lib, count, prec, stop_kkt, steps=1, fp64_sample=8,
               cpu=True, mmin=50, mmax=2000):
     """Batched solve of `count` independent LPs (m log-uniform in [mmin,
@@ -291,7 +371,7 @@ def batch_leg(lib, count, prec, stop_kkt, steps=1, fp64_sample=8,
 
 
 def serve_leg(lib, rate, seconds, stop_kkt, prec=1, seed=9,
-              large_frac=0.01, budget_gb=64.0):
+              large_frac=0.01, budget_gb=64.0, cpu=True):
     """Request loop (continuous batching): Poisson arrivals at `rate` per
     second for `seconds`; 99% LPs built like the batched workload, 1% large
     (m=100,000, n=300,000, 8 nonzeros per column).  Each request's ALM
@@ -340,6 +420,22 @@ def serve_leg(lib, rate, seconds, stop_kkt, prec=1, seed=9,
     srv.close()
     kk = [kkt.kkt(lp, results[i][1], results[i][2])["kkt"]
           for i, lp in enumerate(lps)]
+    cpu_b = None
+    if cpu:
+        try:
+            # inner iterations of one large LP solved alone on the device
+            large_inner = 0
+            if large.any():
+                i = int(np.flatnonzero(large)[0])
+                lib.lib.acd_alm_reset(alms[i])
+                lib.alm_solve_ex(alms[i], 1000, np.zeros(lps[i].ncols),
+                                 np.zeros(lps[i].nrows), precision=prec,
+                                 stop_kkt=stop_kkt * (0.9 if prec else 1.0))
+                large_inner = int(lib.stats().inner_iterations)
+            cpu_b = serve_cpu_baseline(lps, large, arrivals, large_inner,
+                                       procs=min(16, os.cpu_count() or 1))
+        except Exception as e:  # reported, never silently replaced
+            cpu_b = {"value": None, "error": str(e)[-300:]}
     for a in alms:
         lib.alm_free(a)
     for m in mats:
@@ -358,6 +454,7 @@ def serve_leg(lib, rate, seconds, stop_kkt, prec=1, seed=9,
         "large_p50_ms": float(np.percentile(lat[large], 50)) if large.any() else None,
         "kkt_max": float(max(kk)),
         "all_converged": all(r[0] == 0 for r in results.values()),
+        "cpu_baseline": cpu_b,
         "setup_s": setup,
     }
 
@@ -416,7 +513,7 @@ def main():
 
     if args.workload == "serve":
         r = serve_leg(lib, args.rate, args.seconds, args.stop_kkt or 1e-6,
-                      prec=prec)
+                      prec=prec, cpu=not args.no_cpu)
         return ({
             "metric": METRIC, "value": r["solves_per_s"], "unit": "solves/s",
             "n_gpus": 1, "steps": 1, "warmup": 0,
@@ -427,7 +524,7 @@ def main():
                        "stop": "relative KKT <= %g" % (args.stop_kkt or 1e-6),
                        "parallelism": "single GPU"},
             "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
-            "roofline": None, "cpu_baseline": None, "serve": r})
+            "roofline": None, "cpu_baseline": r["cpu_baseline"], "serve": r})
 
     if args.workload == "batch":
         L.acd_synchronize()

@@ -45,7 +45,7 @@ class BatchInfo(C.Structure):
     """struct acd_batch_info (include/acd_gpu.h)."""
     _fields_ = [("device_ms", C.c_double), ("launches", C.c_uint64),
                 ("inner_iterations", C.c_uint64), ("alg_bytes", C.c_double),
-                ("waves_per_lp", C.c_int * 8)]
+                ("waves_per_lp", C.c_int * 8), ("kernel_ms", C.c_double)]
 
 
 class MpsLP(C.Structure):
@@ -232,6 +232,9 @@ class CLib:
                                                 C.c_int, C.POINTER(C.c_double)]),
             ("acd_spmv_set_variant", C.c_int, [C.c_int]),
             ("acd_spmv_autotune", C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
+            ("acd_spmv_rotate_time", C.c_int, [C.POINTER(vp), sz, C.c_int,
+                                               C.POINTER(vp), C.POINTER(vp),
+                                               C.c_int, C.POINTER(C.c_double)]),
             ("acd_last_solve_profile", None, [C.POINTER(C.c_uint64),
                                               C.POINTER(C.c_double), C.c_int]),
             ("acd_last_solve_bytes", None, [C.POINTER(C.c_double),

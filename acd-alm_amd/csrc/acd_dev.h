@@ -109,6 +109,14 @@ int acd_dev_spmv_time_variant(const struct acd_dev_matrix *, int transpose,
                               double * y, const double * x, int reps,
                               int variant, double * OUT_ms);
 int acd_dev_spmv_set_default_variant(int variant);
+/* Mean milliseconds per product over `reps` rounds of k products, product
+ * i on matrix m[i] with d_y[i] <- op d_x[i] (k distinct operators and
+ * vectors, so a working set larger than the device's last-level cache is
+ * streamed from HBM), timed with HIP events on the library stream. */
+int acd_dev_spmv_time_rotate(const struct acd_dev_matrix * const * m,
+                             size_t k, int transpose, double * const * y,
+                             const double * const * x, int reps,
+                             double * OUT_ms);
 
 /* ---- device-resident solver (dev_solver.hip) ---- */
 /* Original-order vectors of a device problem that the host uploads. */
@@ -207,6 +215,9 @@ void acd_dev_last_ctl_phases(double * us4);
 #define ACD_DEV_BATCH_CLASSES 8
 struct acd_batch_stats {
         double device_ms;          /* HIP events around all launches */
+        double kernel_ms;          /* HIP events around each round of
+                                      launches, summed (host gaps between
+                                      rounds excluded) */
         uint64_t launches;
         uint64_t inner_iterations; /* summed over the LPs */
         double alg_bytes;          /* algorithmic bytes, summed */

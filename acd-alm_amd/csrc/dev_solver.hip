@@ -741,7 +741,8 @@ struct Group {
 };
 
 int run_round(int key, Group* grp, uint32_t launches, uint64_t budget,
-              const BatchBufs& b, bool graphs, bool concurrent) {
+              const BatchBufs& b, bool graphs, bool concurrent,
+              hipEvent_t start = nullptr) {
   // One non-empty group (or concurrent = false) runs on the library
   // stream itself: the group streams are created only when sizes are
   // mixed in a batch (extra streams share the device's few hardware
@@ -753,27 +754,33 @@ int run_round(int key, Group* grp, uint32_t launches, uint64_t budget,
   if (groups > 1 && concurrent) {
     gs = group_streams();
     if (!gs) return -1;
-    if (hipEventRecord(gs->fork, S()) != hipSuccess) return -1;
   }
+  // Graphs are captured (host work) before `start` is recorded, so the
+  // round's event interval covers the launches only.
+  for (int i = 0; i < kWpbChoices && graphs; i++) {
+    Group& g = grp[i];
+    if (!g.n || g.have_exec) continue;
+    hipStream_t st = gs ? gs->s[i] : S();
+    hipGraph_t graph;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) !=
+        hipSuccess)
+      return -1;
+    for (uint32_t l = 0; l < launches; l++)
+      launch_batch(key, kWpbOf[i], g.n, g.d_slots, budget, b, st);
+    if (hipStreamEndCapture(st, &graph) != hipSuccess) return -1;
+    const hipError_t e =
+        hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return -1;
+    g.have_exec = true;
+  }
+  if (start && hipEventRecord(start, S()) != hipSuccess) return -1;
+  if (gs && hipEventRecord(gs->fork, S()) != hipSuccess) return -1;
   for (int i = 0; i < kWpbChoices; i++) {
     Group& g = grp[i];
     if (!g.n) continue;
     hipStream_t st = gs ? gs->s[i] : S();
     if (gs && hipStreamWaitEvent(st, gs->fork, 0) != hipSuccess) return -1;
-    if (graphs && !g.have_exec) {
-      hipGraph_t graph;
-      if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) !=
-          hipSuccess)
-        return -1;
-      for (uint32_t l = 0; l < launches; l++)
-        launch_batch(key, kWpbOf[i], g.n, g.d_slots, budget, b, st);
-      if (hipStreamEndCapture(st, &graph) != hipSuccess) return -1;
-      const hipError_t e =
-          hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-      hipGraphDestroy(graph);
-      if (e != hipSuccess) return -1;
-      g.have_exec = true;
-    }
     if (graphs) {
       if (hipGraphLaunch(g.exec, st) != hipSuccess) return -1;
     } else {
@@ -994,9 +1001,12 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   uint64_t *d_done = nullptr, *d_bytes = nullptr, *d_t0 = nullptr;
   uint32_t* d_flags = nullptr;
   int rc = 0;
-  hipEvent_t e0, e1;
+  hipEvent_t e0, e1, r0, r1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
+  hipEventCreate(&r0);
+  hipEventCreate(&r1);
+  double kernel_ms = 0;
   if (!h_args || !h_st || !h_slots || !h_tmp || !h_done || !h_bytes ||
       !cls || !choice ||
       hipMalloc((void**)&d_args, count * sizeof(Args)) != hipSuccess ||
@@ -1070,13 +1080,18 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
     for (int round = 0; !rc && active; round++) {
       const double t = now_s();
       if (run_round(key, grp, kRoundLaunches, kBatchBudget, bufs, graphs,
-                    true) ||
+                    true, r0) ||
+          hipEventRecord(r1, S()) != hipSuccess ||
           hipMemcpyAsync(&done, d_count, sizeof done, hipMemcpyDeviceToHost,
                          S()) != hipSuccess ||
           hipStreamSynchronize(S()) != hipSuccess) {
         acd_dev_record_error(-1, "batch launch");
         rc = -1;
         break;
+      }
+      {
+        float rms = 0;
+        if (hipEventElapsedTime(&rms, r0, r1) == hipSuccess) kernel_ms += rms;
       }
       for (int i = 0; i < kWpbChoices; i++)
         if (grp[i].n) launches += kRoundLaunches;
@@ -1186,6 +1201,7 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
       for (size_t i = 0; i < count; i++)
         stats->waves_per_lp[cls[i]] = kWpbOf[lp_choice(cls[i], forced, c0)];
       stats->device_ms = ms;
+      stats->kernel_ms = kernel_ms;
       stats->launches = launches;
       stats->inner_iterations = inner;
       stats->alg_bytes = (double)total_bytes;
@@ -1193,6 +1209,8 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  hipEventDestroy(r0);
+  hipEventDestroy(r1);
   hipFree(d_args);
   hipFree(d_st);
   hipFree(d_slots);

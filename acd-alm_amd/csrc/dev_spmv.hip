@@ -269,6 +269,36 @@ int acd_dev_spmv_time(const struct acd_dev_matrix* m, int transpose, double* y,
   return acd_dev_spmv_time_variant(m, transpose, y, x, reps, -1, OUT_ms);
 }
 
+int acd_dev_spmv_time_rotate(const struct acd_dev_matrix* const* m, size_t k,
+                             int transpose, double* const* y,
+                             const double* const* x, int reps,
+                             double* OUT_ms) {
+  hipEvent_t a, b;
+  if (k == 0 || hipEventCreate(&a) != hipSuccess ||
+      hipEventCreate(&b) != hipSuccess)
+    return -1;
+  int rc = 0;
+  for (size_t i = 0; i < k && !rc; i++) {  // warm: one product each
+    const struct acd_sell* op = pick(m[i], transpose);
+    rc = launch1(op, y[i], x[i], op->tuned ? op->tuned - 1 : g_default_variant);
+  }
+  hipEventRecord(a, S());
+  for (int r = 0; r < reps && !rc; r++)
+    for (size_t i = 0; i < k && !rc; i++) {
+      const struct acd_sell* op = pick(m[i], transpose);
+      rc = launch1(op, y[i], x[i],
+                   op->tuned ? op->tuned - 1 : g_default_variant);
+    }
+  hipEventRecord(b, S());
+  hipEventSynchronize(b);
+  float ms = 0;
+  hipEventElapsedTime(&ms, a, b);
+  if (OUT_ms) *OUT_ms = reps > 0 ? ms / (reps * (double)k) : 0.0;
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return rc;
+}
+
 int acd_dev_spmv_autotune(struct acd_dev_matrix* m, int reps, int* out) {
   if (reps < 1) reps = 1;
   const size_t nx = (size_t)(m->ncols > m->nrows ? m->ncols : m->nrows) + 1;

@@ -223,6 +223,7 @@ int alm_solve_batch(size_t count, alm_t * const * alms, size_t niter,
                 OUT_info->launches = st.launches;
                 OUT_info->inner_iterations = st.inner_iterations;
                 OUT_info->alg_bytes = st.alg_bytes;
+                OUT_info->kernel_ms = st.kernel_ms;
                 for (int k = 0; k < ACD_BATCH_CLASSES; k++)
                         OUT_info->waves_per_lp[k] = st.waves_per_lp[k];
         }

@@ -43,6 +43,18 @@ int acd_spmv_device_time(const sparse_matrix_t * a, int transpose,
         return acd_dev_spmv_time(&a->dev, transpose, d_y, d_x, reps, OUT_ms);
 }
 
+int acd_spmv_rotate_time(const sparse_matrix_t * const * mats, size_t k,
+                         int transpose, double * const * d_y,
+                         const double * const * d_x, int reps,
+                         double * OUT_ms)
+{
+        if (k == 0 || k > 64) return -1;
+        const struct acd_dev_matrix * dm[64];
+        for (size_t i = 0; i < k; i++) dm[i] = &mats[i]->dev;
+        return acd_dev_spmv_time_rotate(dm, k, transpose, d_y, d_x, reps,
+                                        OUT_ms);
+}
+
 uint64_t acd_matrix_stored_entries(const sparse_matrix_t * a, int transpose)
 {
         return transpose ? a->dev.at.nent : a->dev.a.nent;
@@ -84,7 +96,8 @@ const char * acd_pass_name(int p)
                 "restart_copy", "checkpoint_x", "gradient_pg", "norm_g",
                 "finish", "permute_out", "spg_grad0", "spg_grad", "spg_step",
                 "alm_project", "alm_penalise", "permute_problem",
-                "step_sizes", "alm_violation", "alm_argmax", "alm_update"};
+                "step_sizes", "alm_violation", "alm_argmax", "alm_update",
+                "kkt"};
         if (p < 0 || p >= (int)(sizeof names / sizeof names[0])) return "";
         return names[p];
 }

@@ -265,6 +265,31 @@ static int row_pairs(struct parse * p, char ** tok, int n, int ranges,
         return 0;
 }
 
+/* Interval [lo, hi] of a kept row: E is b (a RANGES value R widens it to
+ * [b, b + |R|] for R > 0, [b - |R|, b] for R < 0), G is [b, b + |R|] or
+ * [b, inf), L is [b - |R|, b] or (-inf, b].  A G or L row with RANGES 0 is
+ * the point b (no slack). */
+static void row_interval(char type, double b, int has_range, double range,
+                         double * lo, double * hi)
+{
+        const double R = fabs(range);
+        switch (type) {
+        case 'E':
+                *lo = *hi = b;
+                if (has_range && range > 0) *hi = b + R;
+                if (has_range && range < 0) *lo = b - R;
+                break;
+        case 'G':
+                *lo = b;
+                *hi = has_range ? b + R : HUGE_VAL;
+                break;
+        default:  /* 'L' */
+                *lo = has_range ? b - R : -HUGE_VAL;
+                *hi = b;
+                break;
+        }
+}
+
 int acd_mps_parse(FILE * stream, struct acd_mps_lp * out)
 {
         struct parse p;
@@ -444,9 +469,10 @@ int acd_mps_parse(FILE * stream, struct acd_mps_lp * out)
                 rowmap[r] = -1;
                 if (p.rtype[r] == 'N') continue;
                 rowmap[r] = (int64_t)m++;
-                const int point = p.rtype[r] == 'E'
-                                  && !(p.has_range[r] && p.range[r] != 0);
-                if (!point) nslack++;
+                double lo_r, hi_r;
+                row_interval(p.rtype[r], p.rhs[r], p.has_range[r],
+                             p.range[r], &lo_r, &hi_r);
+                if (lo_r != hi_r) nslack++;   /* the test of the fill below */
         }
         const size_t n = p.ncol + nslack;
         const size_t nnz = p.nnz + nslack;
@@ -474,23 +500,9 @@ int acd_mps_parse(FILE * stream, struct acd_mps_lp * out)
         for (size_t r = 0; r < p.nrow; r++) {
                 if (rowmap[r] < 0) continue;
                 const size_t i = (size_t)rowmap[r];
-                const double b = p.rhs[r], R = fabs(p.range[r]);
                 double lo_r, hi_r;
-                switch (p.rtype[r]) {
-                case 'E':
-                        lo_r = hi_r = b;
-                        if (p.has_range[r] && p.range[r] > 0) hi_r = b + R;
-                        if (p.has_range[r] && p.range[r] < 0) lo_r = b - R;
-                        break;
-                case 'G':
-                        lo_r = b;
-                        hi_r = p.has_range[r] ? b + R : HUGE_VAL;
-                        break;
-                default:  /* 'L' */
-                        lo_r = p.has_range[r] ? b - R : -HUGE_VAL;
-                        hi_r = b;
-                        break;
-                }
+                row_interval(p.rtype[r], p.rhs[r], p.has_range[r],
+                             p.range[r], &lo_r, &hi_r);
                 if (lo_r == hi_r) {
                         rhs[i] = lo_r;
                         continue;
@@ -512,6 +524,7 @@ int acd_mps_parse(FILE * stream, struct acd_mps_lp * out)
                 k++;
                 s++;
         }
+        if (s != n || k != nnz) FAIL("internal: slack count mismatch");
         out->nrows = m;
         out->ncols = n;
         out->nnz = nnz;

@@ -2,30 +2,43 @@
 """bench.py -- LP solve throughput of the device-resident ALM + APPROX path
 on one MI355X.
 
-A step is one complete ``alm_solve`` (ALM outer loop, APPROX inner loop,
-SpMV) of the set-cover LP relaxation (m = 200,000 rows, n = 1,000,000
-columns plus m slack columns, 0/1 coefficients, 3-8 nonzeros per column,
-c ~ U[1,100], 0 <= x <= 1; seeded synthetic instance), from x = 0 and
-lambda = 0 until the relative KKT residual of (x, lambda), evaluated on
-the device after each outer iteration, is at most 1e-6 (--stop-kkt; 0
-selects the reference's absolute rule, alm.c:303-319, instead).  The matrix,
-the problem vectors and all solver state are resident in HBM before the
-timed region; each step restores the multiplier weights and penalised cost
-(acd_alm_reset) and solves again.
+Default workload (one step = one batched solve): 4,096 independent LPs built
+like the plain random LP (m log-uniform in [50, 2000], n = 2-3 m, 5
+nonzeros per column at distinct uniform rows, values +-U[0.5, 2], b = A x0
+with x0 ~ U[0, 10], c ~ U[-1, 1], 0 <= x <= 10; seeded generator
+acdalm.instances.batch(4096, seed=4)), all solved together by
+alm_solve_batch in mixed precision (FP32 matrix values and per-column
+constants, FP64 iterates, accumulation, multipliers and scalars), from
+x = 0 and lambda = 0 until each LP's relative KKT residual is at most 1e-6
+(checked on the host afterwards).  Matrices, problem vectors and solver
+state are resident in HBM before the timed region; each step restores the
+row weights and penalised cost (acd_alm_reset) and solves all LPs again.
 
-Printed (one JSON line): solves/s over the K timed steps, p50/p99 latency
-per solve, the relative KKT residual of the result (checked <= 1e-6), the
-roofline of the solver's step kernel (algorithmic bytes / kernel time,
-both accumulated by the solver's own control kernel over the timed solves),
-the stand-alone SpMV kernel's HBM throughput on the same matrix (HIP
-events), and the CPU baseline: the reference's thread_pool CPU path
-(oracle/_ref/ref_bench, built from the reference sources) or, where that
-build is absent, the oracle's C restatement, timed on a bounded sample of
-APPROX iterations of the same instance and scaled by the number of inner
-iterations the solve takes.
+Printed (one JSON line on stdout):
+  value        solves/s = LPs x steps / device time of the steps (HIP events
+               around the batched launches; the host-side upload of x0 and
+               lambda0 and the download of the results are outside; the
+               rate including them is `wall_solves_per_s`);
+  p50/p99      per-LP completion time from the start of its batch;
+  roofline     the batched kernel k_batch: algorithmic bytes / the summed
+               HIP-event time of its launch rounds, against 8 TB/s;
+  fp64 ratio   the plain FP64 single-LP form (alm_solve_ex, one LP at a
+               time) on a size-stratified sample of the same LPs;
+  cpu_baseline the oracle (this repository's C restatement of the
+               reference's CPU path, oracle/ora_bench) solving a
+               size-stratified sample LP-parallel on the host cores, to
+               the same KKT rule, and on one thread;
+  spmv         stand-alone SpMV on the set-cover (C1) and multicommodity
+               (C2) operators, rotating over several copies so the working
+               set exceeds the 256 MB last-level cache, with the oracle's
+               threaded SpMV on the host beside it.
+Legs after the timed loop run only while the wall-clock budget
+(--budget-s) allows; a leg that does not fit is reported as skipped.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-                       [--precision mixed|fp64] [--cpu-iters N] [--no-cpu]
+--workload setcover|mcf makes one ALM solve of the set-cover or
+multicommodity LP the step; --workload serve runs the request loop.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload W]
 """
 from __future__ import annotations
 
@@ -47,10 +60,16 @@ METRIC = ("solves/s + p50/p99 latency at 1e-6 rel KKT residual; "
           "SpMV HBM GB/s vs peak")
 HBM_PEAK_GBS = 8000.0
 KKT_TOL = 1e-6
+ORA_BENCH = os.path.join(HERE, "oracle", "_build", "ora_bench")
+T_START = time.perf_counter()
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def elapsed():
+    return time.perf_counter() - T_START
 
 
 class _StdoutToStderr:
@@ -72,203 +91,428 @@ class _StdoutToStderr:
         return False
 
 
-def emit(line):
-    print(json.dumps(line), flush=True)
+def host_cpu():
+    """Host cores this process may use (affinity, capped by the box's
+    declared share in OMP_NUM_THREADS) and the CPU model."""
+    avail = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(avail, share) if share > 0 else avail
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": max(1, threads), "affinity_cpus": avail,
+            "omp_num_threads": share or None, "model": model}
 
 
-def spmv_leg(lib, mat, lp, reps=50):
-    """Stand-alone SpMV (y = A x and y = A' x) on the solver's matrix,
-    timed with HIP events on the library stream, after the autotuner has
-    picked the variant (unroll, rows per wavefront, load kind) per
-    operator."""
-    L = lib.lib
-    tuned = (C.c_int * 2)()
-    if L.acd_spmv_autotune(mat, 10, tuned):
-        raise RuntimeError("SpMV autotune failed: %s" % (lib.last_error(),))
-    out = {"variant_A": int(tuned[0]), "variant_AT": int(tuned[1])}
-    for tr in (0, 1):
-        ny, nx = (lp.ncols, lp.nrows) if tr else (lp.nrows, lp.ncols)
-        dx = L.acd_device_alloc(8 * nx)
-        dy = L.acd_device_alloc(8 * ny)
-        x = np.random.default_rng(7).random(nx)
-        L.acd_copy_to_device(dx, x.ctypes.data, 8 * nx)
-        ms = C.c_double()
-        rc = L.acd_spmv_device_time(mat, tr, dy, dx, reps, C.byref(ms))
-        L.acd_device_free(dx)
-        L.acd_device_free(dy)
-        if rc:
-            raise RuntimeError("SpMV timing failed: %s" % (lib.last_error(),))
-        slices = (ny + 63) // 64
-        alg = lp.nnz * (8 + 4) + 8 * (slices + 1) + 8 * nx + 8 * ny
-        gbs = alg / (ms.value * 1e-3) / 1e9
-        out["AT" if tr else "A"] = {
-            "us": round(ms.value * 1e3, 2), "alg_bytes": int(alg),
-            "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
-    return out
+def stratified(lps, k):
+    """k LPs spread evenly over the batch sorted by size (nonzeros)."""
+    order = sorted(range(len(lps)), key=lambda i: (lps[i].nnz, i))
+    k = min(k, len(order))
+    return [order[(j * len(order)) // k + len(order) // (2 * k)]
+            for j in range(k)]
+
+
+# ---------------------------------------------------------------- CPU legs
+def ora_alm(lps, idx, threads, stop_kkt):
+    """The oracle's ALM (APPROX inner solver, niter 1000, from 0) on the
+    LPs idx, LP-parallel on `threads` host threads; its JSON result."""
+    with tempfile.TemporaryDirectory() as td:
+        paths = []
+        for k, i in enumerate(idx):
+            p = os.path.join(td, "lp%d.lpb" % k)
+            lps[i].write_binary(p)
+            paths.append(p)
+        r = subprocess.run([ORA_BENCH, "alm", str(threads), repr(stop_kkt)]
+                           + paths, capture_output=True, text=True,
+                           timeout=600)
+    if r.returncode:
+        raise RuntimeError("ora_bench alm: " + r.stderr[-300:])
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def batch_cpu_baseline(lps, stop_kkt, cpu, sample=256, sample1=16):
+    """Oracle ALM on a size-stratified sample, LP-parallel on all host
+    threads (solves/s = sample / wall), and on one thread."""
+    idx = stratified(lps, sample)
+    res = ora_alm(lps, idx, cpu["threads"], stop_kkt)
+    idx1 = stratified(lps, sample1)
+    res1 = ora_alm(lps, idx1, 1, stop_kkt)
+    bad = sum(1 for r in res["ret"] if r != 0)
+    return {"value": len(idx) / res["wall_s"], "unit": "solves/s",
+            "cores": cpu["threads"], "kind": "port",
+            "value_1_thread": len(idx1) / res1["wall_s"],
+            "kkt_max": max(res["kkt"]), "not_converged": bad,
+            "cpu_model": cpu["model"], "affinity_cpus": cpu["affinity_cpus"],
+            "sample": ("%d LPs spread over the size-sorted batch, solved by "
+                       "the oracle's ALM (APPROX inner solver, niter 1000) "
+                       "to relative KKT <= %g, one LP per thread at a time "
+                       "on %d threads (%.1f s wall); one thread: %d LPs "
+                       "(%.1f s)" % (len(idx), stop_kkt, cpu["threads"],
+                                     res["wall_s"], len(idx1),
+                                     res1["wall_s"]))}
 
 
 def cpu_spmv(lp, threads, repeat=10):
-    """The reference's sparse_matrix_multiply (A and A', FP64, its thread
-    pool with `threads` workers) on the same matrix, from the build in
-    oracle/_ref; mean seconds per product over `repeat` products."""
-    ref = os.path.join(HERE, "oracle", "_ref", "ref_bench")
-    if not os.path.exists(ref):
-        return {"error": "reference build absent"}
+    """The oracle's CSR products y = A x and y = A' x, rows split over
+    `threads` host threads; mean seconds per product."""
     with tempfile.TemporaryDirectory() as td:
         path = os.path.join(td, "lp.lpb")
         lp.write_binary(path)
-        r = subprocess.run([ref, path, str(threads), "spmv", str(repeat)],
-                           capture_output=True, text=True, timeout=900)
+        r = subprocess.run([ORA_BENCH, "spmv", str(threads), str(repeat),
+                            path], capture_output=True, text=True,
+                           timeout=600)
     if r.returncode:
         return {"error": r.stderr[-300:]}
     res = json.loads(r.stdout.strip().splitlines()[-1])
-    out = {"threads": threads, "repeat": repeat, "kind": "reference"}
+    out = {"threads": threads, "repeat": repeat, "kind": "port"}
     for key, tr in (("A", 0), ("AT", 1)):
         ny, nx = (lp.ncols, lp.nrows) if tr else (lp.nrows, lp.ncols)
-        alg = lp.nnz * (8 + 4) + 8 * ((ny + 63) // 64 + 1) + 8 * nx + 8 * ny
+        alg = spmv_bytes(lp, ny, nx)
         sec = res["s_per_at" if tr else "s_per_a"]
         out[key] = {"us": round(sec * 1e6, 1),
                     "GB/s": round(alg / sec / 1e9, 2)}
     return out
 
 
-def cpu_baseline(lp, inner_iterations, iters, threads):
-    """Bounded CPU sample: `iters` APPROX iterations of the first ALM
-    subproblem on the reference build (or the oracle restatement), scaled to
-    the inner iterations one solve takes."""
-    ref = os.path.join(HERE, "oracle", "_ref", "ref_bench")
-    with tempfile.TemporaryDirectory() as td:
-        if os.path.exists(ref):
-            path = os.path.join(td, "lp.lpb")
-            lp.write_binary(path)
-            r = subprocess.run([ref, path, str(threads), "approx", str(iters)],
-                               capture_output=True, text=True, timeout=900)
-            if r.returncode:
-                raise RuntimeError("ref_bench failed: " + r.stderr[-400:])
-            res = json.loads(r.stdout.strip().splitlines()[-1])
-            spi = res["s_per_iteration"]
-            kind, cores = "reference", threads
-            # the same on one thread (a tenth of the iterations)
-            r1 = subprocess.run([ref, path, "1", "approx",
-                                 str(max(10, iters // 10))],
-                                capture_output=True, text=True, timeout=900)
-            spi1 = (json.loads(r1.stdout.strip().splitlines()[-1])
-                    ["s_per_iteration"] if r1.returncode == 0 else None)
-        else:
-            from pyoracle import Oracle
-            ora = Oracle()
-            m = ora.make_matrix(lp, 0)
-            a = ora.approx_make(m, lp)
-            ora.approx_update(a)
-            t = time.perf_counter()
-            ora.approx_solve(a, np.zeros(lp.ncols), iters)
-            spi = (time.perf_counter() - t) / iters
-            ora.approx_free(a)
-            ora.free_matrix(m)
-            kind, cores = "port", 1
-            spi1 = spi
-    per_solve = spi * inner_iterations
-    return {"value": 1.0 / per_solve, "unit": "solves/s", "cores": cores,
-            "kind": kind,
-            "value_1_thread": (1.0 / (spi1 * inner_iterations)
-                               if spi1 else None),
-            "sample": ("%d APPROX iterations of the first ALM subproblem of "
-                       "the same instance (%.4g s/iteration on %d thread%s), "
-                       "scaled by the %d inner iterations of one solve"
-                       % (iters, spi, cores, "" if cores == 1 else "s",
-                          inner_iterations))}
+# ---------------------------------------------------------------- GPU legs
+def spmv_bytes(lp, ny, nx):
+    """Algorithmic bytes of one FP64 product: each nonzero (8-byte value,
+    4-byte column) once, the slice offsets, x once, y once."""
+    return lp.nnz * (8 + 4) + 8 * ((ny + 63) // 64 + 1) + 8 * nx + 8 * ny
 
 
-def batch_cpu_baseline(lps, sample=32, procs=16):
-    """The reference's CPU path on the batched workload, LP-parallel (one
-    LP per process, pool = NULL, as its thread pool cannot be shared by
-    concurrent solves): `sample` LPs spread over the size-sorted batch,
-    `procs` at a time; solves/s = sample / wall time.  It stops on the
-    reference's absolute rule (its only rule)."""
-    ref = os.path.join(HERE, "oracle", "_ref", "ref_bench")
-    if not os.path.exists(ref):
-        return {"value": None, "error": "reference build absent"}
-    order = sorted(range(len(lps)), key=lambda i: lps[i].nnz)
-    pick = [order[(k * len(order)) // sample] for k in range(sample)]
-    with tempfile.TemporaryDirectory() as td:
-        paths = []
-        for k, i in enumerate(pick):
-            path = os.path.join(td, "lp%d.lpb" % k)
-            lps[i].write_binary(path)
-            paths.append(path)
+def spmv_leg(lib, lp, copies=4, reps=20):
+    """Stand-alone FP64 SpMV, y = A x and y = A' x, on `copies` device
+    copies of the matrix used in rotation (their operators together exceed
+    the 256 MB last-level cache), timed with HIP events on the library
+    stream; and on one copy repeated (cache-assisted when it fits)."""
+    L = lib.lib
+    mats = [lib.make_matrix(lp, 0) for _ in range(copies)]
+    tuned = (C.c_int * 2)()
+    if L.acd_spmv_autotune(mats[0], 5, tuned):
+        raise RuntimeError("SpMV autotune failed: %s" % (lib.last_error(),))
+    for m in mats[1:]:
+        L.acd_spmv_autotune(m, 1, None)
+    out = {"variant_A": int(tuned[0]), "variant_AT": int(tuned[1]),
+           "copies": copies}
+    for tr in (0, 1):
+        ny, nx = (lp.ncols, lp.nrows) if tr else (lp.nrows, lp.ncols)
+        dxs = [L.acd_device_alloc(8 * nx) for _ in mats]
+        dys = [L.acd_device_alloc(8 * ny) for _ in mats]
+        x = np.random.default_rng(7).random(nx)
+        for dx in dxs:
+            L.acd_copy_to_device(dx, x.ctypes.data, 8 * nx)
+        ms = C.c_double()
+        arr_m = (C.c_void_p * copies)(*mats)
+        rc = L.acd_spmv_rotate_time(arr_m, copies, tr,
+                                    (C.c_void_p * copies)(*dys),
+                                    (C.c_void_p * copies)(*dxs), reps,
+                                    C.byref(ms))
+        ms1 = C.c_double()
+        rc = rc or L.acd_spmv_device_time(mats[0], tr, dys[0], dxs[0], reps,
+                                          C.byref(ms1))
+        for p in dxs + dys:
+            L.acd_device_free(p)
+        if rc:
+            raise RuntimeError("SpMV timing failed: %s" % (lib.last_error(),))
+        alg = spmv_bytes(lp, ny, nx)
+        gbs = alg / (ms.value * 1e-3) / 1e9
+        gbs1 = alg / (ms1.value * 1e-3) / 1e9
+        stored = int(L.acd_matrix_stored_entries(mats[0], tr))
+        out["AT" if tr else "A"] = {
+            "us": round(ms.value * 1e3, 2), "alg_bytes": int(alg),
+            "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "working_set_MB": round(copies * (stored * 12 + 8 * (nx + ny))
+                                    / 1e6, 1),
+            "one_copy_us": round(ms1.value * 1e3, 2),
+            "one_copy_GB/s": round(gbs1, 1)}
+    for m in mats:
+        lib.free_matrix(m)
+    return out
+
+
+def fp64_single_leg(lib, lps, alms, idx, stop_kkt):
+    """The plain FP64 GPU form: alm_solve_ex one LP at a time (the
+    library's default engine) to the same KKT target, on the LPs idx."""
+    from acdalm import kkt
+    L = lib.lib
+    t = time.perf_counter()
+    worst = 0.0
+    for i in idx:
+        L.acd_alm_reset(alms[i])
+        rc, x, lam, diag, _ = lib.alm_solve_ex(
+            alms[i], 1000, np.zeros(lps[i].ncols), np.zeros(lps[i].nrows),
+            precision=0, stop_kkt=stop_kkt)
+        if rc != 0:
+            raise RuntimeError("fp64 single-LP solve did not converge")
+        worst = max(worst, kkt.kkt(lps[i], x, lam)["kkt"])
+    sec = time.perf_counter() - t
+    return {"solves_per_s": len(idx) / sec, "lps": len(idx),
+            "seconds": sec, "kkt_max": worst,
+            "sample": "%d LPs spread over the size-sorted batch" % len(idx)}
+
+
+def run_batch(args, lib):
+    from acdalm import instances, kkt
+    L = lib.lib
+    prec = {"mixed": 1, "fp64": 0}[args.precision]
+    count = args.batch_count
+    # With FP32-rounded matrix values the device measures the KKT residual
+    # of the rounded LP; a 10% margin keeps the true LP's residual (checked
+    # on the host) under the target.
+    dev_kkt = args.stop_kkt * (0.9 if prec else 1.0)
+    t0 = time.perf_counter()
+    lps = instances.batch(count, seed=4, mmin=args.batch_mmin,
+                          mmax=args.batch_mmax)
+    mats = [lib.make_matrix(lp, 0) for lp in lps]
+    alms = [lib.alm_make(m, lp) for m, lp in zip(mats, lps)]
+    x0s = [np.zeros(lp.ncols) for lp in lps]
+    l0s = [np.zeros(lp.nrows) for lp in lps]
+    # niter = 0 runs no iteration; it makes the reduced-precision copies.
+    lib.alm_solve_batch(alms, 0, x0s, l0s, precision=prec)
+    setup = time.perf_counter() - t0
+    log("batch setup %.1f s (%d LPs, %d nonzeros)"
+        % (setup, count, sum(lp.nnz for lp in lps)))
+
+    def step():
+        for a in alms:
+            L.acd_alm_reset(a)
         t = time.perf_counter()
-        running, fails = [], 0
-        for path in paths:
-            while len(running) >= procs:
-                fails += running.pop(0).wait() != 0
-            running.append(subprocess.Popen(
-                [ref, path, "1", "alm"], stdout=subprocess.DEVNULL,
-                stderr=subprocess.DEVNULL))
-        for p in running:
-            fails += p.wait() != 0
+        out, done, info = lib.alm_solve_batch(alms, 1000, x0s, l0s,
+                                              precision=prec,
+                                              stop_kkt=dev_kkt)
         wall = time.perf_counter() - t
-    return {"value": sample / wall, "unit": "solves/s", "cores": procs,
-            "kind": "reference", "failed": fails,
-            "sample": ("%d LPs spread over the size-sorted batch, one per "
-                       "process, %d processes at a time, reference stopping "
-                       "rule (%.1f s wall)" % (sample, procs, wall))}
+        bad = [i for i, r in enumerate(out) if r[0] != 0]
+        if bad:
+            raise RuntimeError("batch: %d LPs did not converge" % len(bad))
+        return out, done, info, wall
+
+    for w in range(args.warmup):
+        _, _, info, wall = step()
+        log("warmup %d: %.2f s device, %.2f s wall" % (w, info.device_ms / 1e3,
+                                                       wall))
+    L.acd_synchronize()
+    dev_ms = ker_ms = alg = walls = 0.0
+    launches = inner = 0
+    dones = []
+    for k in range(args.steps):
+        out, done, info, wall = step()
+        dev_ms += info.device_ms
+        ker_ms += info.kernel_ms
+        alg += info.alg_bytes
+        launches += int(info.launches)
+        inner += int(info.inner_iterations)
+        walls += wall
+        dones.append(done)
+        wpl = list(info.waves_per_lp)
+        log("step %d: %.2f s device (%.2f s kernel), %.2f s wall, %d launches"
+            % (k, info.device_ms / 1e3, info.kernel_ms / 1e3, wall,
+               info.launches))
+    L.acd_synchronize()
+    kk = [kkt.kkt(lp, r[1], r[2])["kkt"] for lp, r in zip(lps, out)]
+    done = np.concatenate(dones)
+    value = count * args.steps / (dev_ms * 1e-3)
+    achieved = alg / (ker_ms * 1e-3) / 1e9
+    res = {
+        "value": value, "ms_per_step": dev_ms / args.steps,
+        "p50_ms": float(np.percentile(done, 50)),
+        "p99_ms": float(np.percentile(done, 99)),
+        "kkt_max": float(max(kk)), "kkt_ok": bool(max(kk) <= KKT_TOL),
+        "wall_solves_per_s": count * args.steps / walls,
+        "inner_iterations_per_lp": inner / args.steps / count,
+        "waves_per_lp_by_class": wpl,
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_batch (batched solver kernel, one workgroup per "
+                      "LP, 32 inner iterations per launch)",
+            "launches": launches,
+            "avg_us": round(ker_ms * 1e3 / max(launches, 1), 1),
+            "alg_bytes_per_launch": round(alg / max(launches, 1)),
+            "kernel_ms_per_step": ker_ms / args.steps},
+        "setup_s": setup,
+    }
+    legs = {}
+    budget = args.budget_s
+
+    def leg(name, est_s, fn):
+        if args.no_legs or name in args.skip:
+            legs[name] = {"skipped": "not requested"}
+            return
+        if elapsed() + est_s > budget:
+            legs[name] = {"skipped": "wall budget: %.0f s used of %.0f s, leg "
+                                     "needs ~%.0f s" % (elapsed(), budget,
+                                                        est_s)}
+            return
+        t = time.perf_counter()
+        try:
+            legs[name] = fn()
+        except Exception as e:  # reported, never silently replaced
+            legs[name] = {"error": str(e)[-300:]}
+        log("leg %s: %.1f s" % (name, time.perf_counter() - t))
+
+    cpu = host_cpu()
+    idx64 = stratified(lps, args.fp64_sample)
+    leg("fp64_single_lp", 15, lambda: fp64_single_leg(lib, lps, alms, idx64,
+                                                      args.stop_kkt))
+    if not args.no_cpu:
+        leg("cpu_baseline", 45,
+            lambda: batch_cpu_baseline(lps, args.stop_kkt, cpu))
+    for a in alms:
+        lib.alm_free(a)
+    for m in mats:
+        lib.free_matrix(m)
+
+    def spmv_case(which):
+        lp = instances.set_cover() if which == "setcover" else \
+            instances.multicommodity()
+        r = spmv_leg(lib, lp)
+        r["workload"] = ("set-cover m=%d n=%d nnz=%d" if which == "setcover"
+                         else "multicommodity flow m=%d n=%d nnz=%d") % (
+                             lp.nrows, lp.ncols, lp.nnz)
+        if not args.no_cpu:
+            r["cpu_port"] = cpu_spmv(lp, cpu["threads"])
+        return r
+
+    leg("spmv_mcf", 40, lambda: spmv_case("mcf"))
+    leg("spmv_setcover", 40, lambda: spmv_case("setcover"))
+    f64 = legs.get("fp64_single_lp", {})
+    res["vs_fp64_single_lp"] = (value / f64["solves_per_s"]
+                                if "solves_per_s" in f64 else None)
+    res["legs"] = legs
+    res["cpu_baseline"] = legs.get("cpu_baseline")
+    res["workload"] = ("%d independent LPs, m log-uniform [%d,%d], n=2-3m, "
+                       "5 nnz/column, values +-U[0.5,2]" % (
+                           count, args.batch_mmin, args.batch_mmax))
+    return res
 
 
-def serve_cpu_baseline(lps, large, arrivals, large_inner, procs=16,
-                       sample=32, large_iters=20):
-    """The reference's CPU path replayed on the same arrival trace: a
-    server with `procs` host threads where a small LP takes one thread
-    (pool = NULL) and a large LP all of them (its thread pool).  Service
-    times: each small LP's from a log-log interpolation over `sample`
-    small LPs solved by the reference (ALM, its stopping rule, one
-    process each, `procs` at a time); a large LP's from `large_iters`
-    APPROX iterations on `procs` threads scaled by `large_inner`, the
-    inner iterations the device solve of one large LP took.  FIFO
-    admission; p50/p99 of completion minus arrival."""
-    ref = os.path.join(HERE, "oracle", "_ref", "ref_bench")
-    if not os.path.exists(ref):
-        return {"value": None, "error": "reference build absent"}
+def run_single(args, lib):
+    """One ALM solve of the set-cover or multicommodity LP per step."""
+    from acdalm import instances, kkt
+    L = lib.lib
+    prec = {"mixed": 1, "fp64": 0}[args.precision]
+    t0 = time.perf_counter()
+    if args.workload == "mcf":
+        lp = instances.multicommodity()
+        workload = ("multicommodity flow LP: 100x100 grid, 39,600 arcs, 64 "
+                    "commodities, m=%d n=%d, +-1 A" % (lp.nrows, lp.ncols))
+    else:
+        lp = instances.set_cover()
+        workload = ("set-cover LP relaxation m=200000 n=1000000+200000 "
+                    "slack, 0/1 A, 3-8 nnz/column")
+    mat = lib.make_matrix(lp, 0)
+    alm = lib.alm_make(mat, lp)
+    x0, lam0 = np.zeros(lp.ncols), np.zeros(lp.nrows)
+    # One untimed outer iteration builds the solver graphs and the
+    # reduced-precision copies of the matrix.
+    lib.alm_solve_ex(alm, 1, x0, lam0, precision=prec)
+    setup = time.perf_counter() - t0
+
+    def one():
+        L.acd_alm_reset(alm)
+        t = time.perf_counter()
+        r = lib.alm_solve_ex(alm, args.outer, x0, lam0,
+                             log="stderr" if args.log else False,
+                             precision=prec, stop_kkt=args.stop_kkt)
+        return (time.perf_counter() - t,) + tuple(r)
+
+    for w in range(args.warmup):
+        dt, rc, *_ = one()
+        log("warmup %d: %.2f s rc=%d" % (w, dt, rc))
+    names = [L.acd_pass_name(j).decode() for j in range(32)]
+    step_us, bytes_, lat, objs = np.zeros(32), np.zeros(32), [], []
+    launches = 0
+    L.acd_synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        dt, rc, x, lam, diag, _ = one()
+        lat.append(dt)
+        st = lib.stats()
+        for name, (cnt, sus, by) in lib.profile_bytes().items():
+            i = names.index(name)
+            step_us[i] += sus
+            bytes_[i] += by
+            launches += cnt
+        objs.append(diag[0])
+        log("step %d: %.2f s rc=%d inner=%d outer=%d obj=%.12g"
+            % (k, dt, rc, st.inner_iterations, st.outer_iterations, diag[0]))
+    L.acd_synchronize()
+    total = time.perf_counter() - t_start
+    res_k = kkt.kkt(lp, x, lam)
+    achieved = bytes_.sum() / (step_us.sum() * 1e-6) / 1e9
+    top = np.argsort(-step_us)[:3]
+    lat_ms = np.array(lat) * 1e3
+    lib.alm_free(alm)
+    lib.free_matrix(mat)
+    return {
+        "value": args.steps / total, "ms_per_step": total / args.steps * 1e3,
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "kkt_max": res_k["kkt"], "kkt_ok": bool(rc == 0 and
+                                                 res_k["kkt"] <= KKT_TOL),
+        "objective": objs[-1], "deterministic": len(set(objs)) == 1,
+        "inner_iterations": int(st.inner_iterations),
+        "outer_iterations": int(st.outer_iterations),
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_step (solver step kernel, all passes)",
+            "launches": int(launches),
+            "avg_us": round(step_us.sum() / max(launches, 1), 2),
+            "alg_bytes_per_launch": round(bytes_.sum() / max(launches, 1)),
+            "by_pass": {names[i]: {
+                "GB/s": round(bytes_[i] / (step_us[i] * 1e-6) / 1e9, 1),
+                "share": round(step_us[i] / step_us.sum(), 3)}
+                for i in top if step_us[i] > 0}},
+        "cpu_baseline": None, "setup_s": setup, "workload": workload,
+        "nnz": int(lp.nnz), "m": int(lp.nrows), "n": int(lp.ncols),
+    }
+
+
+def serve_cpu_baseline(lps, large, arrivals, large_inner, cpu, sample=32,
+                       large_iters=20):
+    """The oracle replayed on the same arrival trace: a server with
+    `threads` host threads where a small LP takes one thread and a large LP
+    all of them.  Service times: each small LP's from a log-log
+    interpolation over `sample` small LPs solved by the oracle (ALM to the
+    same KKT rule, LP-parallel); a large LP's from `large_iters` serial
+    APPROX iterations scaled by `large_inner` (the inner iterations the
+    device took for one large LP) and divided by the thread count (a
+    generous, perfectly parallel CPU).  FIFO admission; p50/p99 of
+    completion minus arrival."""
+    threads = cpu["threads"]
     small = [i for i in range(len(lps)) if not large[i]]
-    order = sorted(small, key=lambda i: lps[i].nnz)
-    pick = [order[(k * len(order)) // sample] for k in range(min(sample, len(order)))]
-    times = {}
-    with tempfile.TemporaryDirectory() as td:
-        jobs = []
-        for k, i in enumerate(pick):
-            path = os.path.join(td, "lp%d.lpb" % k)
-            lps[i].write_binary(path)
-            jobs.append((i, path))
-        running = []
-        for i, path in jobs:
-            while len(running) >= procs:
-                j, t0, pr = running.pop(0)
-                pr.wait()
-                times[j] = time.perf_counter() - t0
-            running.append((i, time.perf_counter(), subprocess.Popen(
-                [ref, path, "1", "alm"], stdout=subprocess.DEVNULL,
-                stderr=subprocess.DEVNULL)))
-        for j, t0, pr in running:
-            pr.wait()
-            times[j] = time.perf_counter() - t0
-        spi = None
-        big = [i for i in range(len(lps)) if large[i]]
-        if big:
+    pick = [small[j] for j in stratified([lps[i] for i in small], sample)]
+    res = ora_alm(lps, pick, threads, KKT_TOL)
+    times = dict(zip(pick, res["seconds"]))
+    spi = None
+    big = [i for i in range(len(lps)) if large[i]]
+    if big:
+        with tempfile.TemporaryDirectory() as td:
             path = os.path.join(td, "large.lpb")
             lps[big[0]].write_binary(path)
-            r = subprocess.run([ref, path, str(procs), "approx",
-                                str(large_iters)], capture_output=True,
-                               text=True, timeout=900)
-            if r.returncode == 0:
-                spi = json.loads(r.stdout.strip().splitlines()[-1])[
-                    "s_per_iteration"]
-    xs = np.log([lps[i].nnz for i in sorted(times, key=lambda i: lps[i].nnz)])
-    ys = np.log([times[i] for i in sorted(times, key=lambda i: lps[i].nnz)])
+            r = subprocess.run([ORA_BENCH, "approx", str(large_iters), path],
+                               capture_output=True, text=True, timeout=600)
+        if r.returncode == 0:
+            spi = json.loads(r.stdout.strip().splitlines()[-1])[
+                "s_per_iteration"] / threads
+    srt = sorted(times, key=lambda i: lps[i].nnz)
+    xs = np.log([lps[i].nnz for i in srt])
+    ys = np.log([max(times[i], 1e-6) for i in srt])
     service = np.array([
         (spi * large_inner if spi else np.nan) if large[i]
         else float(np.exp(np.interp(np.log(lps[i].nnz), xs, ys)))
         for i in range(len(lps))])
-    # replay: `procs` workers, FIFO; a large LP waits for all of them
-    free_at = np.zeros(procs)
+    free_at = np.zeros(threads)
     done = np.zeros(len(lps))
     for i in np.argsort(arrivals, kind="stable"):
         if large[i]:
@@ -281,97 +525,20 @@ def serve_cpu_baseline(lps, large, arrivals, large_inner, procs=16,
         done[i] = start + service[i]
     lat = (done - np.asarray(arrivals)) * 1e3
     return {"value": float(len(lps) / done.max()), "unit": "solves/s",
-            "cores": procs, "kind": "reference",
+            "cores": threads, "kind": "port",
             "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)),
-            "large_service_s": (spi * large_inner) if spi else None,
             "sample": ("trace replay on %d host threads; small-LP service "
-                       "times from %d reference ALM solves (one thread "
-                       "each, median %.3g s), large-LP service time %.4g "
-                       "s/iteration x %d inner iterations on %d threads"
-                       % (procs, len(times), float(np.median(list(
-                           times.values()))), spi or float("nan"),
-                          large_inner, procs))}
-
-
-def batch_leg(lib, count, prec, stop_kkt, steps=1, fp64_sample=8,
-              cpu=True, mmin=50, mmax=2000):
-    """Batched solve of `count` independent LPs (m log-uniform in [mmin,
-    mmax], default [50, 2000], n = 2-3 m, 5 nonzeros per column, random
-    +-U[0.5, 2] values; seeded),
-    all device-resident, to the relative KKT target.  Also times the plain
-    FP64 single-LP form (alm_solve_ex, one LP at a time) on the first
-    `fp64_sample` LPs for the ratio."""
-    from acdalm import instances, kkt
-    L = lib.lib
-    t0 = time.perf_counter()
-    lps = instances.batch(count, seed=4, mmin=mmin, mmax=mmax)
-    mats = [lib.make_matrix(lp, 0) for lp in lps]
-    alms = [lib.alm_make(m, lp) for m, lp in zip(mats, lps)]
-    setup = time.perf_counter() - t0
-    x0s = [np.zeros(lp.ncols) for lp in lps]
-    l0s = [np.zeros(lp.nrows) for lp in lps]
-    dev_ms, done_all, kkts, inner, alg = 0.0, [], [], 0, 0.0
-    for _ in range(steps):
-        for a in alms:
-            L.acd_alm_reset(a)
-        # With FP32-rounded matrix values the device measures the KKT
-        # residual of the rounded LP; a 10% margin keeps the true LP's
-        # residual (checked below on the host) under the target.
-        out, done, info = lib.alm_solve_batch(
-            alms, 1000, x0s, l0s, precision=prec,
-            stop_kkt=stop_kkt * (0.9 if prec else 1.0))
-        dev_ms += info.device_ms
-        done_all.append(done)
-        inner += int(info.inner_iterations)
-        alg += info.alg_bytes
-        wpb = list(info.waves_per_lp)
-        bad = [i for i, r in enumerate(out) if r[0] != 0]
-        if bad:
-            raise RuntimeError("batch: %d LPs did not converge" % len(bad))
-    kkts = [kkt.kkt(lp, r[1], r[2])["kkt"] for lp, r in zip(lps, out)]
-    done = np.concatenate(done_all)
-    # the plain FP64 form on a sample, same target
-    t = time.perf_counter()
-    for a, lp in zip(alms[:fp64_sample], lps[:fp64_sample]):
-        L.acd_alm_reset(a)
-        lib.alm_solve_ex(a, 1000, np.zeros(lp.ncols), np.zeros(lp.nrows),
-                         precision=0, stop_kkt=stop_kkt)
-    single = fp64_sample / (time.perf_counter() - t)
-    for a in alms:
-        lib.alm_free(a)
-    for m in mats:
-        lib.free_matrix(m)
-    cpu_b = None
-    if cpu:
-        try:
-            cpu_b = batch_cpu_baseline(lps, procs=min(16, os.cpu_count() or 1))
-        except Exception as e:  # reported, never silently replaced
-            cpu_b = {"value": None, "error": str(e)[-300:]}
-    rate = count * steps / (dev_ms * 1e-3)
-    gbs = alg / (dev_ms * 1e-3) / 1e9
-    return {
-        "workload": "%d independent LPs, m log-uniform [%d,%d], n=2-3m, "
-                    "5 nnz/column" % (count, mmin, mmax),
-        "precision": "mixed" if prec == 1 else "fp64",
-        "solves_per_s": rate, "device_ms_per_batch": dev_ms / steps,
-        "p50_ms": float(np.percentile(done, 50)),
-        "p99_ms": float(np.percentile(done, 99)),
-        "kkt_max": float(max(kkts)), "inner_iterations": inner // steps,
-        "waves_per_lp_by_class": wpb,
-        "roofline": {"kernel": "k_batch", "achieved": round(gbs, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4)},
-        "fp64_single_lp_solves_per_s": single,
-        "fp64_single_lp_sample": fp64_sample,
-        "vs_fp64_single_lp": rate / single,
-        "cpu_baseline": cpu_b,
-        "setup_s": setup,
-    }
+                       "times from %d oracle ALM solves to KKT %g; large-LP "
+                       "service %s" % (threads, len(pick), KKT_TOL,
+                                       "%.4g s/iteration (serial / %d) x %d "
+                                       "inner iterations" % (
+                                           spi * threads, threads,
+                                           large_inner) if spi else "n/a"))}
 
 
 def serve_leg(lib, rate, seconds, stop_kkt, prec=1, seed=9,
-              large_frac=0.01, budget_gb=64.0, cpu=True):
+              large_frac=0.01, budget_gb=64.0, cpu=None):
     """Request loop (continuous batching): Poisson arrivals at `rate` per
     second for `seconds`; 99% LPs built like the batched workload, 1% large
     (m=100,000, n=300,000, 8 nonzeros per column).  Each request's ALM
@@ -397,8 +564,9 @@ def serve_leg(lib, rate, seconds, stop_kkt, prec=1, seed=9,
     mats = [lib.make_matrix(lp, 0) for lp in lps]
     alms = [lib.alm_make(m, lp) for m, lp in zip(mats, lps)]
     setup = time.perf_counter() - t0
+    dev_kkt = stop_kkt * (0.9 if prec else 1.0)
     srv = lib.server(int(budget_gb * 2**30), slots=2048, large_rows=50000,
-                     precision=prec, stop_kkt=stop_kkt * (0.9 if prec else 1.0))
+                     precision=prec, stop_kkt=dev_kkt)
     lat = np.zeros(len(lps))
     results = {}
     nxt = 0
@@ -421,19 +589,17 @@ def serve_leg(lib, rate, seconds, stop_kkt, prec=1, seed=9,
     kk = [kkt.kkt(lp, results[i][1], results[i][2])["kkt"]
           for i, lp in enumerate(lps)]
     cpu_b = None
-    if cpu:
+    if cpu is not None:
         try:
-            # inner iterations of one large LP solved alone on the device
             large_inner = 0
             if large.any():
                 i = int(np.flatnonzero(large)[0])
                 lib.lib.acd_alm_reset(alms[i])
                 lib.alm_solve_ex(alms[i], 1000, np.zeros(lps[i].ncols),
                                  np.zeros(lps[i].nrows), precision=prec,
-                                 stop_kkt=stop_kkt * (0.9 if prec else 1.0))
+                                 stop_kkt=dev_kkt)
                 large_inner = int(lib.stats().inner_iterations)
-            cpu_b = serve_cpu_baseline(lps, large, arrivals, large_inner,
-                                       procs=min(16, os.cpu_count() or 1))
+            cpu_b = serve_cpu_baseline(lps, large, arrivals, large_inner, cpu)
         except Exception as e:  # reported, never silently replaced
             cpu_b = {"value": None, "error": str(e)[-300:]}
     for a in alms:
@@ -445,250 +611,103 @@ def serve_leg(lib, rate, seconds, stop_kkt, prec=1, seed=9,
         "workload": "Poisson arrivals at %g/s for %g s: %d requests (%d "
                     "large m=100000 n=300000)" % (rate, seconds, len(lps),
                                                    int(large.sum())),
-        "solves_per_s": len(lps) / end, "offered_per_s": rate,
-        "makespan_s": end,
+        "value": len(lps) / end, "ms_per_step": end * 1e3,
+        "offered_per_s": rate, "makespan_s": end,
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
         "small_p50_ms": float(np.percentile(lat[small], 50)) if small.any() else None,
         "small_p99_ms": float(np.percentile(lat[small], 99)) if small.any() else None,
         "large_p50_ms": float(np.percentile(lat[large], 50)) if large.any() else None,
-        "kkt_max": float(max(kk)),
+        "kkt_max": float(max(kk)), "kkt_ok": bool(max(kk) <= KKT_TOL),
         "all_converged": all(r[0] == 0 for r in results.values()),
-        "cpu_baseline": cpu_b,
-        "setup_s": setup,
+        "roofline": None, "cpu_baseline": cpu_b, "setup_s": setup,
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["batch", "setcover", "mcf",
+                                           "serve"], default="batch")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--cpu-iters", type=int, default=300)
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--stop-kkt", type=float, default=1e-6,
+    ap.add_argument("--stop-kkt", type=float, default=KKT_TOL,
                     help="stop once the relative KKT residual is at most "
-                         "this (0: the reference's absolute rule)")
-    ap.add_argument("--log", action="store_true",
-                    help="stream the ALM log (one line per outer iteration) "
-                         "to stderr")
+                         "this (0: the reference's absolute rule; single "
+                         "LP workloads only)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="batch: only the timed steps")
+    ap.add_argument("--skip", default="",
+                    help="batch: comma-separated legs to leave out "
+                         "(fp64_single_lp, cpu_baseline, spmv_mcf, "
+                         "spmv_setcover)")
+    ap.add_argument("--budget-s", type=float, default=480.0,
+                    help="wall-clock budget: legs after the timed steps run "
+                         "only while they fit")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-mmin", type=int, default=50)
+    ap.add_argument("--batch-mmax", type=int, default=2000)
+    ap.add_argument("--fp64-sample", type=int, default=16)
     ap.add_argument("--outer", type=int, default=1000,
-                    help="ALM outer-iteration cap (the reference uses 1000)")
-    ap.add_argument("--workload", choices=["setcover", "mcf", "batch", "serve"],
-                    default="setcover")
+                    help="setcover/mcf: ALM outer-iteration cap")
+    ap.add_argument("--log", action="store_true",
+                    help="setcover/mcf: stream the ALM log to stderr")
     ap.add_argument("--rate", type=float, default=2000.0,
                     help="serve: offered requests per second")
     ap.add_argument("--seconds", type=float, default=1.0,
                     help="serve: length of the arrival window")
-    ap.add_argument("--batch-count", type=int, default=4096)
-    ap.add_argument("--batch-mmin", type=int, default=50,
-                    help="batched LPs: smallest row count (log-uniform)")
-    ap.add_argument("--batch-mmax", type=int, default=2000,
-                    help="batched LPs: largest row count")
-    ap.add_argument("--no-batch", action="store_true",
-                    help="setcover workload: skip the batched-solve leg")
-    ap.add_argument("--copy-check", type=int, default=0, metavar="BYTES",
-                    help="first run one device-to-device copy of BYTES "
-                         "(calibrates hardware byte counters)")
     args = ap.parse_args()
+    args.skip = set(filter(None, args.skip.split(",")))
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU "
-              "path (each LP solve runs on one device).", file=sys.stderr)
+              "path (every LP solve runs on one device; how LPs and one "
+              "large LP would shard is described in DESIGN.md).",
+              file=sys.stderr)
         sys.exit(2)
 
-    from acdalm import capi, instances, kkt
+    from acdalm import capi
     lib = capi.load()
-    L = lib.lib
-    prec = {"mixed": 1, "fp64": 0}[args.precision]
-
-    if args.copy_check:
-        src = L.acd_device_alloc(args.copy_check)
-        dst = L.acd_device_alloc(args.copy_check)
-        L.acd_copy_device(dst, src, args.copy_check)
-        L.acd_device_free(src)
-        L.acd_device_free(dst)
-
     if args.workload == "serve":
-        r = serve_leg(lib, args.rate, args.seconds, args.stop_kkt or 1e-6,
-                      prec=prec, cpu=not args.no_cpu)
-        return ({
-            "metric": METRIC, "value": r["solves_per_s"], "unit": "solves/s",
-            "n_gpus": 1, "steps": 1, "warmup": 0,
-            "ms_per_step": r["makespan_s"] * 1e3,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64", "data": "synthetic (seeded generator)",
-            "config": {"workload": r["workload"], "precision": args.precision,
-                       "stop": "relative KKT <= %g" % (args.stop_kkt or 1e-6),
-                       "parallelism": "single GPU"},
-            "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
-            "roofline": None, "cpu_baseline": r["cpu_baseline"], "serve": r})
-
-    if args.workload == "batch":
-        L.acd_synchronize()
-        t = time.perf_counter()
-        b = batch_leg(lib, args.batch_count, prec, args.stop_kkt,
-                      steps=args.steps, cpu=not args.no_cpu,
-                      mmin=args.batch_mmin, mmax=args.batch_mmax)
-        total = time.perf_counter() - t
-        return ({
-            "metric": METRIC, "value": b["solves_per_s"], "unit": "solves/s",
-            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": b["device_ms_per_batch"],
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64", "data": "synthetic (seeded generator)",
-            "config": {"workload": b["workload"], "precision": args.precision,
-                       "stop": "relative KKT <= %g" % args.stop_kkt,
-                       "parallelism": "single GPU"},
-            "p50_ms": b["p50_ms"], "p99_ms": b["p99_ms"],
-            "roofline": dict(b["roofline"], bound="hbm", traffic=None),
-            "cpu_baseline": b["cpu_baseline"], "batch": b, "wall_s": total})
-
-    t0 = time.perf_counter()
-    if args.workload == "mcf":
-        lp = instances.multicommodity()
-        workload = ("multicommodity flow LP: 100x100 grid, 39,600 arcs, 64 "
-                    "commodities, m=%d n=%d, +-1 A" % (lp.nrows, lp.ncols))
+        r = serve_leg(lib, args.rate, args.seconds, args.stop_kkt,
+                      prec={"mixed": 1, "fp64": 0}[args.precision],
+                      cpu=None if args.no_cpu else host_cpu())
+        steps, warmup = 1, 0
+    elif args.workload == "batch":
+        r = run_batch(args, lib)
+        steps, warmup = args.steps, args.warmup
     else:
-        lp = instances.set_cover()
-        workload = ("set-cover LP relaxation m=200000 n=1000000+200000 "
-                    "slack, 0/1 A, 3-8 nnz/column")
-    mat = lib.make_matrix(lp, 0)
-    alm = lib.alm_make(mat, lp)
-    x0 = np.zeros(lp.ncols)
-    lam0 = np.zeros(lp.nrows)
-    # Untimed priming: one outer iteration builds the solver graphs and the
-    # reduced-precision copies of the matrix.
-    lib.alm_solve_ex(alm, 1, x0, lam0, precision=prec)
-    setup_s = time.perf_counter() - t0
-    log("setup %.1f s (m=%d n=%d nnz=%d)" % (setup_s, lp.nrows, lp.ncols, lp.nnz))
-
-    def one_solve():
-        L.acd_alm_reset(alm)
-        t = time.perf_counter()
-        rc, x, lam, diag, _ = lib.alm_solve_ex(
-            alm, args.outer, x0, lam0, log="stderr" if args.log else False,
-            precision=prec, stop_kkt=args.stop_kkt)
-        return time.perf_counter() - t, rc, x, lam, diag
-
-    for w in range(args.warmup):
-        dt, rc, *_ = one_solve()
-        log("warmup %d: %.2f s rc=%d" % (w, dt, rc))
-
-    L.acd_synchronize()
-    lat = []
-    step_us = np.zeros(32)
-    bytes_ = np.zeros(32)
-    launches = 0
-    inner = outer = 0
-    device_ms = 0.0
-    objs = []
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        dt, rc, x, lam, diag = one_solve()
-        lat.append(dt)
-        st = lib.stats()
-        inner, outer = st.inner_iterations, st.outer_iterations
-        device_ms += st.device_ms
-        for name, (cnt, sus, by) in lib.profile_bytes().items():
-            i = [L.acd_pass_name(j).decode() for j in range(32)].index(name)
-            step_us[i] += sus
-            bytes_[i] += by
-            launches += cnt
-        objs.append(diag[0])
-        log("step %d: %.2f s rc=%d inner=%d outer=%d obj=%.12g"
-            % (k, dt, rc, inner, outer, diag[0]))
-    L.acd_synchronize()
-    total = time.perf_counter() - t_start
-
-    res = kkt.kkt(lp, x, lam)
-    kkt_ok = bool(rc == 0 and res["kkt"] <= KKT_TOL)
-    lat_ms = np.array(lat) * 1e3
-    spmv = spmv_leg(lib, mat, lp)
-    cpu_threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    if not args.no_cpu:
-        spmv["cpu_reference"] = cpu_spmv(lp, cpu_threads)
-
-    achieved = bytes_.sum() / (step_us.sum() * 1e-6) / 1e9
-    names = [L.acd_pass_name(j).decode() for j in range(32)]
-    top = np.argsort(-step_us)[:3]
-    roofline = {
-        "bound": "hbm", "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-        "kernel": "k_step (solver step kernel, all passes)",
-        "launches": int(launches),
-        "avg_us": round(step_us.sum() / max(launches, 1), 2),
-        "alg_bytes_per_launch": round(bytes_.sum() / max(launches, 1)),
-        "by_pass": {names[i]: {
-            "GB/s": round(bytes_[i] / (step_us[i] * 1e-6) / 1e9, 1),
-            "share": round(step_us[i] / step_us.sum(), 3)}
-            for i in top if step_us[i] > 0},
-    }
-    # The stand-alone SpMV on the other headline matrix: the set-cover A is
-    # bound by random gathers, the multicommodity operators stream.
-    spmv_other = None
-    if args.workload == "setcover" and not args.no_batch:
-        try:
-            lp2 = instances.multicommodity()
-            mat2 = lib.make_matrix(lp2, 0)
-            spmv_other = dict(spmv_leg(lib, mat2, lp2),
-                              workload="multicommodity flow LP, m=%d n=%d "
-                              "nnz=%d" % (lp2.nrows, lp2.ncols, lp2.nnz))
-            lib.free_matrix(mat2)
-            if not args.no_cpu:
-                spmv_other["cpu_reference"] = cpu_spmv(lp2, cpu_threads)
-        except Exception as e:  # reported, never silently replaced
-            spmv_other = {"error": str(e)[-300:]}
-    batch = None
-    if not args.no_batch:
-        try:
-            batch = batch_leg(lib, args.batch_count, 1, args.stop_kkt or 1e-6,
-                              cpu=not args.no_cpu)
-        except Exception as e:  # reported, never silently replaced
-            batch = {"error": str(e)[-300:]}
-    cpu = None
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        try:
-            cpu = cpu_baseline(lp, inner, args.cpu_iters, threads)
-        except Exception as e:  # reported, never silently replaced
-            cpu = {"value": None, "error": str(e)[-300:]}
-
+        r = run_single(args, lib)
+        steps, warmup = args.steps, args.warmup
+    mixed = args.precision == "mixed"
     line = {
-        "metric": METRIC,
-        "value": args.steps / total,
-        "unit": "solves/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": total / args.steps * 1e3,
+        "metric": METRIC, "value": r["value"], "unit": "solves/s",
+        "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded generator)",
-        "config": {"workload": workload,
-                   "precision": args.precision,
+        "dtype": "f32 storage / f64 compute" if mixed else "f64",
+        "dtype_detail": ("matrix values and per-column constants (c, l, u, "
+                         "step sizes) stored in FP32; iterates, products, "
+                         "accumulation, multipliers, norms and the theta "
+                         "recurrence in FP64" if mixed else
+                         "FP64 everywhere"),
+        "data": "synthetic (seeded generator, acdalm.instances)",
+        "config": {"workload": r["workload"], "precision": args.precision,
                    "stop": "relative KKT <= %g" % args.stop_kkt
                            if args.stop_kkt > 0 else "reference rule",
-                   "nnz": int(lp.nnz), "m": int(lp.nrows), "n": int(lp.ncols),
                    "parallelism": "single GPU"},
-        "p50_ms": float(np.percentile(lat_ms, 50)),
-        "p99_ms": float(np.percentile(lat_ms, 99)),
-        "kkt": res["kkt"], "kkt_ok": kkt_ok,
-        "objective": objs[-1], "deterministic": len(set(objs)) == 1,
-        "inner_iterations": int(inner), "outer_iterations": int(outer),
-        "solver_device_ms": device_ms / args.steps,
-        "roofline": roofline,
-        "spmv": spmv,
-        "spmv_mcf": spmv_other,
-        "batch": batch,
-        "cpu_baseline": cpu,
-        "setup_s": setup_s,
+        "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+        "roofline": r["roofline"], "cpu_baseline": r["cpu_baseline"],
+        "wall_s": elapsed(),
     }
+    line.update({k: v for k, v in r.items() if k not in line and
+                 k not in ("workload", "value")})
     return line
 
 
 if __name__ == "__main__":
     with _StdoutToStderr():
         result = main()
-    emit(result)
+    print(json.dumps(result), flush=True)

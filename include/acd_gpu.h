@@ -59,6 +59,16 @@ enum { ACD_SPMV_VARIANTS = 9 };
  * (optional, int[2]) receives the choices for A and A'. */
 int acd_spmv_autotune(sparse_matrix_t *, int reps, int * OUT_variants);
 
+/* Mean milliseconds per product over `reps` rounds of k products: in
+ * each round, d_y[i] <- op(mats[i]) d_x[i] for i = 0..k-1, timed with HIP
+ * events on the library stream.  With k distinct matrices whose operators
+ * together exceed the device's last-level cache (256 MB on MI355X), every
+ * product streams its matrix from HBM.  k <= 64. */
+int acd_spmv_rotate_time(const sparse_matrix_t * const * mats, size_t k,
+                         int transpose, double * const * d_y,
+                         const double * const * d_x, int reps,
+                         double * OUT_ms_per_launch);
+
 /* Stored entries of the operator (padding to the slice length included). */
 uint64_t acd_matrix_stored_entries(const sparse_matrix_t *, int transpose);
 
@@ -130,6 +140,10 @@ struct acd_batch_info {
          * the last one; ACD_BATCH_WAVES=1|2|4 forces one size.  Results do
          * not depend on it. */
         int waves_per_lp[ACD_BATCH_CLASSES];
+        /* Device time of the batched-kernel launches: HIP events around
+         * each round of launches on the library stream, summed, so the
+         * host's work between rounds is excluded (device_ms includes it). */
+        double kernel_ms;
 };
 /* alm_solve_ex on `count` independent ALM objects at once: one workgroup
  * per LP, every LP on the device together, repeated launches of 32 inner

@@ -35,6 +35,7 @@ void ora_alm_default_options(struct ora_alm_options * o)
         o->precision_floor = 1e-7;
         o->min_delta = 1e-11;
         o->inner = ORA_INNER_APPROX;
+        o->stop_kkt = 0;
 }
 
 /* alm.c:59-83 */
@@ -198,6 +199,46 @@ static int iter(struct alm_state * s, ora_alm_t * a, double * x,
         return 0;
 }
 
+/* The device library's relative KKT residual (oracle.h), in plain
+ * sequential sums. */
+double ora_alm_kkt(ora_alm_t * a, const double * x, const double * lambda)
+{
+        const size_t m = a->nrhs, n = a->nvars;
+        const double * b = a->approx->rhs, * c = a->linear;
+        const double * lo = a->approx->lower, * up = a->approx->upper;
+        double * r = calloc(m + 1, sizeof(double));
+        double * atl = calloc(n + 1, sizeof(double));
+        ora_sparse_matrix_multiply(r, m, a->matrix, x, n, 0);
+        ora_sparse_matrix_multiply(atl, n, a->matrix, lambda, m, 1);
+        double rr = 0, bb = 0, bl = 0;
+        for (size_t i = 0; i < m; i++) {
+                const double ri = r[i] - b[i];
+                rr += ri * ri;
+                bb += b[i] * b[i];
+                bl += b[i] * lambda[i];
+        }
+        double px = 0, inf2 = 0, bnd = 0, cc = 0, cx = 0;
+        for (size_t j = 0; j < n; j++) {
+                const double d = c[j] - atl[j];
+                const double pr = fmin(fmax(x[j] - d, lo[j]), up[j]);
+                const double dp = fmax(d, 0), dn = fmax(-d, 0);
+                const int lo_inf = !isfinite(lo[j]), up_inf = !isfinite(up[j]);
+                const double inf = (lo_inf ? dp : 0) + (up_inf ? dn : 0);
+                px += (x[j] - pr) * (x[j] - pr);
+                inf2 += inf * inf;
+                bnd += (lo_inf ? 0 : lo[j] * dp) - (up_inf ? 0 : up[j] * dn);
+                cc += c[j] * c[j];
+                cx += c[j] * x[j];
+        }
+        free(r);
+        free(atl);
+        const double dual = bl + bnd;
+        const double rp = sqrt(rr) / (1 + sqrt(bb));
+        const double rd = (sqrt(px) + sqrt(inf2)) / (1 + sqrt(cc));
+        const double rg = fabs(cx - dual) / (1 + fabs(cx) + fabs(dual));
+        return fmax(rp, fmax(rd, rg));
+}
+
 /* alm.c:289-322 */
 int ora_alm_solve(ora_alm_t * a, size_t niter, double * x, size_t nvars,
                   double * lambda, size_t ncons, FILE * log, double * diag,
@@ -217,6 +258,13 @@ int ora_alm_solve(ora_alm_t * a, size_t niter, double * x, size_t nvars,
         for (size_t i = 0; i < niter; i++) {
                 int status = iter(&s, a, x, lambda, log, i + 1, &pg,
                                   &max_viol, &o);
+                if (o.stop_kkt > 0) {
+                        if (ora_alm_kkt(a, x, lambda) <= o.stop_kkt) {
+                                ret = 0;
+                                break;
+                        }
+                        continue;
+                }
                 if ((max_viol < o.stop_violation) && (status || (pg < o.stop_pg))) {
                         ret = 0;
                         break;

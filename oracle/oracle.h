@@ -105,8 +105,22 @@ struct ora_alm_options {
         double precision_floor;
         double min_delta;
         int inner;
+        /* > 0: stop once the relative KKT residual of (x, lambda) is at
+         * most this after an outer iteration, instead of the constants
+         * above (the stopping rule of the device library's stop_kkt). */
+        double stop_kkt;
 };
 void ora_alm_default_options(struct ora_alm_options *);
+/* Relative KKT residual max(r_p, r_d, r_gap) of (x, lambda) for the LP of
+ * an ALM object, with d = c - A'lambda:
+ *   r_p = |Ax - b| / (1 + |b|),
+ *   r_d = (|x - clamp(x - d, l, u)| + |dual infeasibility|) / (1 + |c|),
+ *   r_gap = |c'x - dual| / (1 + |c'x| + |dual|),
+ *   dual = b'lambda + sum(l d+) - sum(u d-) over finite bounds,
+ * where a reduced cost pushing against an infinite bound is dual
+ * infeasibility.  The reference defines no such measure (its rule is
+ * absolute, alm.c:303-319); this is the one the device library stops on. */
+double ora_alm_kkt(ora_alm_t *, const double * x, const double * lambda);
 ora_alm_t * ora_alm_make(ora_matrix_t * constraints, size_t nrhs,
                          const double * rhs, size_t nvars,
                          const double * linear, const double * lower,

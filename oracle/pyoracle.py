@@ -28,7 +28,7 @@ sz = C.c_size_t
 class AlmOptions(C.Structure):
     _fields_ = [("stop_violation", C.c_double), ("stop_pg", C.c_double),
                 ("precision_floor", C.c_double), ("min_delta", C.c_double),
-                ("inner", C.c_int)]
+                ("inner", C.c_int), ("stop_kkt", C.c_double)]
 
 
 def _d(a):
@@ -98,6 +98,7 @@ class Oracle:
             "ora_alm_solve": (C.c_int, [vp, sz, dp, sz, dp, sz, vp, dp,
                                         C.POINTER(AlmOptions)]),
             "ora_alm_default_options": (None, [C.POINTER(AlmOptions)]),
+            "ora_alm_kkt": (C.c_double, [vp, dp, dp]),
             "ora_set_reverse_sums": (None, [C.c_int]),
         }
         for name, (res, args) in sig.items():
@@ -234,6 +235,10 @@ class Oracle:
 
     def alm_free(self, a):
         return self.lib.ora_alm_free(a)
+
+    def alm_kkt(self, a, x, lam):
+        x, lam = _f64(x), _f64(lam)
+        return float(self.lib.ora_alm_kkt(a, _d(x), _d(lam)))
 
     def options(self, **kw):
         o = AlmOptions()

@@ -28,6 +28,23 @@ Fixtures (numpy .npz, float64 unless noted):
   cd_<case>.npz     cd_solve after srandom(1) (the as-shipped inner solver).
   lp_optimum.npz    optimal objectives from scipy.optimize.linprog (HiGHS)
                     for the small LPs, an independent pin of the optimum.
+  batch_sample.npz  64 LPs spread over the size-sorted batched workload
+                    (acdalm.instances.batch(4096, seed=4)): their indices and
+                    digests, the reference's alm_solve (APPROX inner solver,
+                    its stopping rule) return code and objective, and the
+                    HiGHS optimum of each.
+  alm_c0_tight.npz  the oracle's alm_solve of config0(1) with tightened
+                    stopping constants (1e-9, 1e-9, precision floor 1e-12):
+                    x, lambda, diagnosis, and the HiGHS optimum.  The
+                    reference hard-codes its constants (alm.c:244, :306-308),
+                    so the tight run comes from the oracle, which is pinned
+                    bit for bit to the reference at the default constants
+                    (alm_c0.npz, tests/test_oracle.py).
+
+    python tests/golden/make_golden.py [part ...]
+
+regenerates only the named parts (spmv, solve, cd, optimum, batch_sample,
+c0_tight); no argument regenerates all of them.
 """
 import ctypes
 import os
@@ -72,7 +89,67 @@ def save(name, **arrays):
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
-def main():
+def stratified(lps, k):
+    """k LPs spread evenly over the batch sorted by size (nonzeros)."""
+    order = sorted(range(len(lps)), key=lambda i: (lps[i].nnz, i))
+    return [order[(j * len(order)) // k + len(order) // (2 * k)]
+            for j in range(k)]
+
+
+def highs(lp):
+    from scipy.optimize import linprog
+    from scipy.sparse import coo_matrix
+    A = coo_matrix((lp.vals, (lp.rows.astype(np.int64),
+                              lp.cols.astype(np.int64))),
+                   shape=(lp.nrows, lp.ncols)).tocsr()
+    bounds = list(zip(lp.lower, [None if u == np.inf else u
+                                 for u in lp.upper]))
+    res = linprog(lp.linear, A_eq=A, b_eq=lp.rhs, bounds=bounds,
+                  method="highs")
+    assert res.status == 0, res.message
+    return float(res.fun)
+
+
+def batch_sample(ref):
+    lps = I.batch(4096, seed=4)
+    idx = stratified(lps, 64)
+    rc, obj, viol, opt, dig = [], [], [], [], []
+    for k, i in enumerate(idx):
+        lp = lps[i]
+        m = ref.make_matrix(lp, 0)
+        a = ref.alm_make(m, lp)
+        r, x, lam, diag, _ = ref.alm_solve(a, 1000, np.zeros(lp.ncols),
+                                           np.zeros(lp.nrows))
+        ref.alm_free(a)
+        ref.free_matrix(m)
+        rc.append(r)
+        obj.append(diag[0])
+        viol.append(diag[1])
+        opt.append(highs(lp))
+        dig.append(lp.digest())
+        print("batch sample %d/%d: LP %d m=%d rc=%d obj=%.12g highs=%.12g"
+              % (k + 1, len(idx), i, lp.nrows, r, diag[0], opt[-1]))
+    save("batch_sample.npz", idx=np.array(idx), digest=np.array(dig),
+         ref_rc=np.array(rc), ref_obj=np.array(obj),
+         ref_max_viol=np.array(viol), highs_obj=np.array(opt))
+
+
+def c0_tight(ora):
+    lp = I.config0(1)
+    m = ora.make_matrix(lp, 0)
+    a = ora.alm_make(m, lp)
+    rc, x, lam, diag, _ = ora.alm_solve(a, 5000, np.zeros(lp.ncols),
+                                        np.zeros(lp.nrows), stop_violation=1e-9,
+                                        stop_pg=1e-9, precision_floor=1e-12)
+    ora.alm_free(a)
+    ora.free_matrix(m)
+    save("alm_c0_tight.npz", digest=np.array(lp.digest()), rc=np.array(rc),
+         x=x, lam=lam, diag=diag, highs_obj=np.array(highs(lp)))
+
+
+def main(parts=None):
+    parts = set(parts or ["spmv", "solve", "cd", "optimum", "batch_sample",
+                          "c0_tight"])
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"],
                    check=True, stdout=subprocess.DEVNULL)
     ref = CLib(pyoracle.REF_APPROX_SO, gpu_extras=False)
@@ -80,8 +157,12 @@ def main():
     ora = pyoracle.Oracle()
     sys.path.insert(0, os.path.dirname(HERE))
     from test_solver_gpu import logs_match
+    if "batch_sample" in parts:
+        batch_sample(ref)
+    if "c0_tight" in parts:
+        c0_tight(ora)
     rng = np.random.default_rng(2024)
-    for name, lp in spmv_cases().items():
+    for name, lp in (spmv_cases() if "spmv" in parts else {}).items():
         out = {"digest": np.array(lp.digest())}
         x = rng.standard_normal(lp.ncols)
         y = rng.standard_normal(lp.nrows)
@@ -101,7 +182,7 @@ def main():
             ref.free_matrix(m)
         save(f"spmv_{name}.npz", **out)
 
-    for name, lp in solve_cases().items():
+    for name, lp in (solve_cases() if "solve" in parts else {}).items():
         m = ref.make_matrix(lp, 0)
         a = ref.approx_make(m, lp)
         mo = ora.make_matrix(lp, 0)
@@ -157,6 +238,13 @@ def main():
         ref.alm_free(alm)
         ref.free_matrix(m)
 
+    if "cd" in parts:
+        cd_fixture(ref)
+    if "optimum" in parts:
+        optimum_fixture()
+
+
+def cd_fixture(ref):
     # the as-shipped inner solver on a small problem
     lp = solve_cases()["small"]
     m = ref.make_matrix(lp, 0)
@@ -169,6 +257,8 @@ def main():
     ref.approx_free(a)
     ref.free_matrix(m)
 
+
+def optimum_fixture():
     from scipy.optimize import linprog
     opt = {}
     for name, lp in solve_cases().items():
@@ -185,4 +275,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

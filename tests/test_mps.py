@@ -85,6 +85,26 @@ def test_parse_errors(host, tmp_path, text, message):
         host.mps_parse(str(path))
 
 
+def test_zero_ranges_on_inequality_rows_make_equalities(host, tmp_path):
+    """A G or L row with RANGES 0 is the interval [b, b]: an equality with
+    no slack column (the counts of columns and entries follow)."""
+    path = tmp_path / "r0.mps"
+    path.write_text(
+        "NAME R0\nROWS\n N OBJ\n G RG\n L RL\n G RS\nCOLUMNS\n"
+        "    X OBJ 1 RG 1\n    X RL 2\n    Y RL 1 RS 3\n"
+        "RHS\n    RHS RG 4 RL 5\n    RHS RS 6\n"
+        "RANGES\n    RNG RG 0 RL 0\nENDATA\n")
+    p = host.mps_parse(str(path))
+    # RG: x = 4; RL: 2x + y = 5; RS (no range): 3y - s = 6, s >= 0
+    assert (p["nrows"], p["ncols"], p["nstructural"]) == (3, 3, 2)
+    coo = sorted(zip(p["rows"].tolist(), p["cols"].tolist(),
+                     p["values"].tolist()))
+    assert coo == [(0, 0, 1.0), (1, 0, 2.0), (1, 1, 1.0), (2, 1, 3.0),
+                   (2, 2, -1.0)]
+    assert p["rhs"].tolist() == [4.0, 5.0, 6.0]
+    assert p["lower"].tolist()[2] == 0.0 and p["upper"].tolist()[2] == INF
+
+
 def test_parse_round_trip_of_a_random_lp(host, tmp_path):
     lp = I.random_lp(300, 700, 5, 44)
     path = str(tmp_path / "r.mps")

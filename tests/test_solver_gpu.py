@@ -80,6 +80,20 @@ def problems(gpu):
         gpu.free_matrix(m)
 
 
+def host_diag(lp, x, offset):
+    """Objective value c'x + 1/2 |Ax - b|^2 + offset (unit weights), the
+    gradient norm |A'(Ax - b) + c| and the projected-gradient norm
+    |x - clamp(x - g, l, u)| at x, recomputed on the host
+    (approx/approx.c:258-278, gradient_value.inc:111-181)."""
+    r = np.bincount(lp.rows, weights=lp.vals * x[lp.cols],
+                    minlength=lp.nrows) - lp.rhs
+    g = np.bincount(lp.cols, weights=lp.vals * r[lp.rows],
+                    minlength=lp.ncols) + lp.linear
+    value = lp.linear @ x + 0.5 * (r @ r) + offset
+    pg = np.linalg.norm(x - np.clip(x - g, lp.lower, lp.upper))
+    return value, np.linalg.norm(g), pg
+
+
 @pytest.mark.parametrize("name", sorted(SOLVE))
 @pytest.mark.parametrize("solver", ["approx", "mpg"])
 def test_inner_solver_matches_reference(gpu, problems, name, solver):
@@ -90,20 +104,35 @@ def test_inner_solver_matches_reference(gpu, problems, name, solver):
         print(f"[solver] {solver} {name} niter={cap}", flush=True)
         rc, x, diag, log = fn(a, np.zeros(lp.ncols), int(cap), log=True,
                               period=10, offset=0.5)
-        # Tolerance: 1e-9, or a thousand times how far the reference's own
-        # thread-pool run moves from its serial run (make_golden.py).
-        xtol = max(1e-9, 1e3 * float(f[f"{solver}_pooldev_{cap}"]))
-        dtol = max(1e-9, 1e3 * float(f[f"{solver}_pooldiagdev_{cap}"]))
+        want_x, want_d = f[f"{solver}_x_{cap}"], f[f"{solver}_diag_{cap}"]
         assert rc == int(f[f"{solver}_rc_{cap}"])
-        assert close(x, f[f"{solver}_x_{cap}"], xtol), cap
-        assert close(diag[:4], f[f"{solver}_diag_{cap}"][:4], dtol), (diag, f[f"{solver}_diag_{cap}"])
-        assert diag[4] == f[f"{solver}_diag_{cap}"][4]
+        assert diag[4] == want_d[4]
+        # The outputs describe the returned point: value, |g| and the
+        # projected gradient recomputed on the host from x.
+        v, ng, pg = host_diag(lp, x, 0.5)
+        assert abs(diag[0] - v) <= 1e-9 * (1 + abs(v)), (diag[0], v)
+        assert abs(diag[1] - ng) <= 1e-7 * (1 + ng), (diag[1], ng)
+        assert abs(diag[2] - pg) <= 1e-7 * (1 + pg), (diag[2], pg)
+        # How far the reference's own result moves under another summation
+        # order (its 4-worker pool; the oracle summing in reverse), from
+        # make_golden.py.  Where it moves by at most 1e-9 the trajectory
+        # does not depend on the order and the device must land within
+        # 1e3 times that (at least 1e-9, never above 1e-6).  Beyond that
+        # the reference itself is order-sensitive (SPG's Barzilai-Borwein
+        # step amplifies rounding: on c0 at 250 iterations the pool run's
+        # objective differs entirely), and the checks above are what can
+        # be asserted.
+        xdev = float(f[f"{solver}_pooldev_{cap}"])
+        ddev = float(f[f"{solver}_pooldiagdev_{cap}"])
+        if xdev <= 1e-9:
+            xtol = min(1e-6, max(1e-9, 1e3 * xdev))
+            dtol = min(1e-6, max(1e-9, 1e3 * ddev))
+            assert close(x, want_x, xtol), cap
+            assert close(diag[:4], want_d[:4], dtol), (diag, want_d)
         # Logs must agree wherever the trajectory does not depend on the
         # summation order: the reference's own thread-pool run prints the
-        # same log and neither it nor the reversed-order oracle moves x
-        # (see tests/golden/make_golden.py).
-        if (bool(f[f"{solver}_poolstable_{cap}"])
-                and float(f[f"{solver}_pooldev_{cap}"]) <= 1e-12):
+        # same log and neither it nor the reversed-order oracle moves x.
+        if bool(f[f"{solver}_poolstable_{cap}"]) and xdev <= 1e-12:
             want = str(f[f"{solver}_log_{cap}"])
             assert logs_match(log, want), log_diff(log, want)
 

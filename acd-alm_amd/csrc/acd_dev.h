@@ -109,6 +109,17 @@ int acd_dev_spmv_time_variant(const struct acd_dev_matrix *, int transpose,
                               double * y, const double * x, int reps,
                               int variant, double * OUT_ms);
 int acd_dev_spmv_set_default_variant(int variant);
+/* Allocation-free product (no hipMalloc/hipFree): host x is staged in and
+ * host y out through a fixed device array of 2 x ACD_NOALLOC_MAX doubles
+ * that the library holds per device.  Returns 0, or -1 (error recorded)
+ * when a length exceeds ACD_NOALLOC_MAX or does not match the operator.
+ * Calls that overlap on one device share the array and race. */
+#define ACD_NOALLOC_MAX 4194304u
+int acd_dev_spmv_noalloc(const struct acd_dev_matrix *, int transpose,
+                         double * y, size_t ny, const double * x, size_t nx);
+/* Looks up the staging array's device address for the current device
+ * (once per device); 0 on success. */
+int acd_dev_scratch_setup(void);
 /* Mean milliseconds per product over `reps` rounds of k products, product
  * i on matrix m[i] with d_y[i] <- op d_x[i] (k distinct operators and
  * vectors, so a working set larger than the device's last-level cache is

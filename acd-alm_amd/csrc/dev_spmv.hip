@@ -213,6 +213,38 @@ int launch1(const struct acd_sell* op, double* y, const double* x, int variant) 
 // the set-cover and multicommodity operators measured on MI355X.
 int g_default_variant = 1;
 
+// Staging memory of the allocation-free product (acd_dev_spmv_noalloc):
+// x in the first ACD_NOALLOC_MAX doubles, y in the second.  Each device
+// holds its own copy from the library's load until the process ends.  In
+// host code `scratch` names a host-side stand-in, never device memory: its
+// device address is looked up per device (scratch_ptr) and only that
+// address is passed to copies and kernels.
+static __device__ __align__(256) double scratch[2 * (size_t)ACD_NOALLOC_MAX];
+
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices];
+
+// Device address of this device's `scratch`, looked up once per device
+// with that device current (the library's device).
+double* scratch_ptr() {
+  int dev = acd_dev_device();
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    int cur = -1;
+    hipGetDevice(&cur);
+    if (cur != dev) hipSetDevice(dev);
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev && cur >= 0) hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+      acd_dev_record_error((int)e, "hipGetSymbolAddress(scratch)");
+      return nullptr;
+    }
+    g_scratch[dev] = (double*)p;
+  }
+  return g_scratch[dev];
+}
+
 }  // namespace
 
 extern "C" {
@@ -333,6 +365,33 @@ int acd_dev_spmv_autotune(struct acd_dev_matrix* m, int reps, int* out) {
   hipFree(x);
   hipFree(y);
   return rc;
+}
+
+int acd_dev_scratch_setup(void) { return scratch_ptr() ? 0 : -1; }
+
+int acd_dev_spmv_noalloc(const struct acd_dev_matrix* m, int transpose,
+                         double* y, size_t ny, const double* x, size_t nx) {
+  const struct acd_sell* op = pick(m, transpose);
+  if (nx > ACD_NOALLOC_MAX || ny > ACD_NOALLOC_MAX) {
+    acd_dev_record_error(-1, "allocation-free product: vector longer than "
+                             "the 4,194,304-element staging maximum");
+    return -1;
+  }
+  if (ny != op->nrows || nx != op->ncols) {
+    acd_dev_record_error(-1, "allocation-free product: length mismatch");
+    return -1;
+  }
+  if (ny == 0) return 0;
+  double* sx = scratch_ptr();
+  if (!sx) return -1;
+  double* sy = sx + ACD_NOALLOC_MAX;
+  // Every y[0..ny) is written by the product (each row of every slice,
+  // empty rows included) and x is fully uploaded, so nothing left in the
+  // staging memory by an earlier call is read.
+  if (acd_dev_h2d(sx, x, nx * sizeof(double))) return -1;
+  if (launch1(op, sy, sx, op->tuned ? op->tuned - 1 : g_default_variant))
+    return -1;
+  return acd_dev_d2h(y, sy, ny * sizeof(double));
 }
 
 int acd_dev_spmv_set_default_variant(int variant) {

@@ -22,6 +22,9 @@ void sparse_matrix_init(void)
 {
         if (acd_dev_init())
                 acd_fail("sparse_matrix_init: no usable HIP device");
+        /* device address of the allocation-free product's staging array */
+        if (acd_dev_scratch_setup())
+                acd_fail("sparse_matrix_init: staging array");
 }
 
 /* spmv/spmv.c:27-92 */
@@ -168,6 +171,19 @@ int sparse_matrix_multiply(double * OUT_y, size_t ny,
             || acd_dev_d2h(OUT_y, m->stage_y, ny * sizeof(double)))
                 acd_fail("sparse_matrix_multiply");
         return 0;
+}
+
+int acd_sparse_matrix_multiply_noalloc(double * OUT_y, size_t ny,
+                                       const sparse_matrix_t * a,
+                                       const double * x, size_t nx,
+                                       int transpose)
+{
+        if (a == NULL || (ny && OUT_y == NULL) || (nx && x == NULL)) {
+                acd_dev_record_error(-1, "allocation-free product: NULL "
+                                     "argument");
+                return -1;
+        }
+        return acd_dev_spmv_noalloc(&a->dev, transpose, OUT_y, ny, x, nx);
 }
 
 /* spmv/spmv.c:195-233 */

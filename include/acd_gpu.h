@@ -59,6 +59,24 @@ enum { ACD_SPMV_VARIANTS = 9 };
  * (optional, int[2]) receives the choices for A and A'. */
 int acd_spmv_autotune(sparse_matrix_t *, int reps, int * OUT_variants);
 
+/* sparse_matrix_multiply without any device allocation or free: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  OUT_y[ny] <- op(a) x[nx] (op = A' when
+ * transpose), host arrays, staged through a fixed device array of
+ * 2 x 4,194,304 doubles (64 MiB) that the library holds on each device it
+ * runs on until the process ends (sparse_matrix_multiply instead
+ * allocates staging buffers per matrix on first use).  Returns 0; or -1,
+ * with acd_last_error() describing it, when nx or ny exceeds 4,194,304,
+ * a length does not match the operator, or an argument is NULL.  The
+ * result is that of sparse_matrix_multiply bit for bit.  Calls that
+ * overlap on one device (several host threads) share the array and
+ * race; serialise them. */
+int acd_sparse_matrix_multiply_noalloc(double * OUT_y, size_t ny,
+                                       const sparse_matrix_t * a,
+                                       const double * x, size_t nx,
+                                       int transpose);
+
 /* Mean milliseconds per product over `reps` rounds of k products: in
  * each round, d_y[i] <- op(mats[i]) d_x[i] for i = 0..k-1, timed with HIP
  * events on the library stream.  With k distinct matrices whose operators

@@ -45,7 +45,8 @@ class BatchInfo(C.Structure):
     """struct acd_batch_info (include/acd_gpu.h)."""
     _fields_ = [("device_ms", C.c_double), ("launches", C.c_uint64),
                 ("inner_iterations", C.c_uint64), ("alg_bytes", C.c_double),
-                ("waves_per_lp", C.c_int * 8), ("kernel_ms", C.c_double)]
+                ("waves_per_lp", C.c_int * 8), ("kernel_ms", C.c_double),
+                ("lp_ops_ms", C.c_double), ("lp_ctl_ms", C.c_double)]
 
 
 class MpsLP(C.Structure):

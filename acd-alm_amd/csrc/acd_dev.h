@@ -24,6 +24,9 @@ void acd_dev_free(void * p);
 int acd_dev_h2d(void * dst, const void * src, size_t bytes);
 int acd_dev_d2h(void * dst, const void * src, size_t bytes);
 int acd_dev_d2d(void * dst, const void * src, size_t bytes);
+/* dst <- src by a plain HIP copy kernel (16 bytes per lane, grid-stride;
+ * bytes and both addresses multiples of 16), on the library stream. */
+int acd_dev_copy_kernel(void * dst, const void * src, size_t bytes);
 int acd_dev_memset(void * dst, int byte, size_t bytes);
 int acd_dev_sync(void);
 void * acd_dev_stream(void);            /* hipStream_t */
@@ -88,7 +91,7 @@ int acd_dev_sell_make_bf16(struct acd_sell *);
 /* y = op x on device pointers (op = A' when transpose). */
 int acd_dev_spmv(const struct acd_dev_matrix *, int transpose,
                  double * y, const double * x);
-enum { ACD_DEV_SPMV_VARIANTS = 9 };
+enum { ACD_DEV_SPMV_VARIANTS = 13 };
 /* Times every SpMV variant (unroll x slices per wavefront x load kind) on
  * both operators of m, `reps` products each, and keeps the fastest per
  * operator; out[0], out[1] receive the choices for A and A'.  Variants
@@ -232,13 +235,19 @@ struct acd_batch_stats {
         uint64_t launches;
         uint64_t inner_iterations; /* summed over the LPs */
         double alg_bytes;          /* algorithmic bytes, summed */
+        /* LP-milliseconds (summed over the LPs' workgroups) spent in ops
+         * (passes and their reductions) and in the control continuation */
+        double lp_ops_ms, lp_ctl_ms;
         /* workgroup size / 64 used for each size class (0: no LP) */
         int waves_per_lp[ACD_DEV_BATCH_CLASSES];
 };
+/* sink (optional, with q->log set and count == 1): log events of the LP,
+ * replayed between launches and at the end, in order. */
 int acd_dev_solve_batch(struct acd_dev_lp * const * lps, size_t count,
                         const struct acd_solve_request * q,
                         struct acd_solve_result * res, double * done_ms,
-                        struct acd_batch_stats * stats);
+                        struct acd_batch_stats * stats, acd_event_sink sink,
+                        void * ctx);
 
 /* ---- continuous batching (dev_solver.hip) ---- */
 /* A pool of `cap` slots for the batched kernel: LPs are admitted into free

@@ -102,6 +102,35 @@ int acd_dev_d2h(void* dst, const void* src, size_t bytes) {
          note(hipStreamSynchronize(g_stream), "sync after D2H");
 }
 
+}  // extern "C"
+
+namespace {
+// Plain device copy, 16 bytes per lane per step, grid-stride (the
+// reference kernel of known traffic for calibrating byte counters).
+__global__ void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                         size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" {
+
+int acd_dev_copy_kernel(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return 0;
+  if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) {
+    acd_dev_record_error(-1, "copy kernel: size or alignment not 16 bytes");
+    return -1;
+  }
+  const size_t n = bytes / 16;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  k_copy16<<<(unsigned)blocks, 256, 0, g_stream>>>((uint4*)dst,
+                                                   (const uint4*)src, n);
+  return note(hipGetLastError(), "copy kernel");
+}
+
 int acd_dev_d2d(void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return 0;
   return note(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice,

@@ -193,15 +193,19 @@ struct BlockG {
   uint32_t lane, gwave, nwaves;
 };
 
-// Two waves per SIMD for every workgroup size (the kernel needs ~240
-// VGPRs; without the bound the 1- and 2-wave variants take the whole
-// register file and run one wave per SIMD).
+// The control continuation is called out of line (Ctl::advance<true>):
+// inlined, its temporaries took the kernel to ~240 VGPRs (2 waves per
+// SIMD); called, the kernel needs ~150 and runs 3 waves per SIMD, i.e. 3
+// LP workgroups of 4 waves per CU instead of 2 (measured on 4,096 LPs:
+// 344 against 336 solves/s; bounding it to 4 waves per SIMD spills and
+// ran 316).
+constexpr bool kBatchCtlCall = true;
 template <class VI, class VC, class MT, int WPB>
 __global__ void __launch_bounds__(64 * WPB)
-__attribute__((amdgpu_waves_per_eu(2)))
+__attribute__((amdgpu_waves_per_eu(3)))
 k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
         uint64_t budget, uint32_t* done_count, uint64_t* t_done,
-        uint64_t* bytes, uint32_t* flags) {
+        uint64_t* bytes, uint32_t* flags, uint64_t* ticks) {
   static_assert(kVWaves % WPB == 0, "virtual waves per physical wave");
   __shared__ Args la;
   __shared__ State ls;
@@ -222,6 +226,9 @@ k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
   if (ls.pc == PC_DONE || ls.error) return;
   const uint64_t stop_at = ls.inner_total + budget;
   uint64_t moved = 0;
+  // Wall-clock ticks of this launch spent in ops (passes + reductions)
+  // and in the control continuation, kept by thread 0.
+  uint64_t t_ops = 0, t_ctl = 0, t_mark = wall_clock64();
   for (uint32_t guard = 0; guard < kBatchOps; guard++) {
     if (ls.op != OP_NONE) {
       const uint32_t mm = ls.op_maxmask;
@@ -252,9 +259,13 @@ k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
       __syncthreads();
     }
     if (tid == 0) {
+      const uint64_t t1 = wall_clock64();
+      t_ops += t1 - t_mark;
       if (ls.op != OP_NONE) moved += (uint64_t)alg_bytes(ls.op, ls.op_flags, la);
       Ctl c{ls, la, res};
-      c.advance();
+      c.advance<kBatchCtlCall>();
+      t_mark = wall_clock64();
+      t_ctl += t_mark - t1;
     }
     __syncthreads();
     if (ls.pc == PC_DONE || ls.error || ls.op == OP_NONE ||
@@ -268,6 +279,8 @@ k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
   }
   if (tid == 0) {
     bytes[lp] += moved;
+    ticks[2 * lp] += t_ops;
+    ticks[2 * lp + 1] += t_ctl;
     if (ls.pc == PC_DONE || ls.error) {
       t_done[lp] = wall_clock64();
       flags[lp] = 1u;
@@ -610,6 +623,7 @@ struct BatchBufs {
   uint64_t* t_done;
   uint64_t* bytes;
   uint32_t* flags;
+  uint64_t* ticks;  // [2 per LP]: ops, control
 };
 
 template <int WPB>
@@ -617,7 +631,7 @@ void launch_batch_w(int key, uint32_t nblocks, const uint32_t* slots,
                     uint64_t budget, const BatchBufs& b, hipStream_t st) {
   const dim3 grid(nblocks), block(64 * WPB);
 #define ACD_BATCH_ARGS \
-  b.args, b.st, slots, budget, b.done, b.t_done, b.bytes, b.flags
+  b.args, b.st, slots, budget, b.done, b.t_done, b.bytes, b.flags, b.ticks
   switch (key) {
     case K_MIX_F32:
       k_batch<double, float, float, WPB><<<grid, block, 0, st>>>(
@@ -823,6 +837,34 @@ int upload_groups(const uint32_t* live, uint32_t n, const uint8_t* choice,
   return 0;
 }
 
+// Replays the log events a batch of one has written since the last drain
+// (its state record at d_st) and resets the ring, so a solve paused on a
+// full ring goes on at the next launch.
+int drain_events(acd_dev_lp* lp, State* d_st, acd_event_sink sink,
+                 void* ctx) {
+  State hs;
+  if (hipMemcpy(&hs, d_st, sizeof hs, hipMemcpyDeviceToHost) != hipSuccess)
+    return -1;
+  if (!hs.events) return 0;
+  const uint64_t ne = hs.events < kEventCap ? hs.events : kEventCap;
+  acd_event* buf = (acd_event*)malloc(sizeof(acd_event) * ne);
+  if (!buf || hipMemcpy(buf, lp->ev, sizeof(acd_event) * ne,
+                        hipMemcpyDeviceToHost) != hipSuccess) {
+    free(buf);
+    return -1;
+  }
+  for (uint64_t i = 0; i < ne; i++) sink(ctx, &buf[i]);
+  free(buf);
+  const uint64_t zero = 0;
+  const int32_t zero32 = 0;
+  if (hipMemcpy(&d_st->events, &zero, sizeof zero, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(&d_st->events_full, &zero32, sizeof zero32,
+                hipMemcpyHostToDevice) != hipSuccess)
+    return -1;
+  return 0;
+}
+
 // Batches of at least kTuneMinLps LPs tune on their own first rounds
 // (each candidate configuration runs one round, the fastest is kept) and
 // again whenever the number of unfinished LPs has halved, down to
@@ -963,8 +1005,11 @@ int acd_dev_solve(struct acd_dev_lp* lp, const struct acd_solve_request* q,
 int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
                         const struct acd_solve_request* q,
                         struct acd_solve_result* res, double* done_ms,
-                        struct acd_batch_stats* stats) {
+                        struct acd_batch_stats* stats, acd_event_sink sink,
+                        void* ctx) {
   if (count == 0) return 0;
+  // A log is replayed for a batch of one only (the LP's event ring).
+  const bool logged = sink && q->log && count == 1;
   if (count > 0xffffffffu) return -1;
   // One kernel instantiation per launch: every LP runs in the same key
   // (BF16 values only when all of them are exact there).
@@ -1000,6 +1045,7 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   uint32_t *d_slots = nullptr, *d_count = nullptr;
   uint64_t *d_done = nullptr, *d_bytes = nullptr, *d_t0 = nullptr;
   uint32_t* d_flags = nullptr;
+  uint64_t* d_ticks = nullptr;
   int rc = 0;
   hipEvent_t e0, e1, r0, r1;
   hipEventCreate(&e0);
@@ -1016,7 +1062,9 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
       hipMalloc((void**)&d_done, count * sizeof(uint64_t)) != hipSuccess ||
       hipMalloc((void**)&d_bytes, count * sizeof(uint64_t)) != hipSuccess ||
       hipMalloc((void**)&d_t0, sizeof(uint64_t)) != hipSuccess ||
-      hipMalloc((void**)&d_flags, count * sizeof(uint32_t)) != hipSuccess) {
+      hipMalloc((void**)&d_flags, count * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void**)&d_ticks, 2 * count * sizeof(uint64_t)) !=
+          hipSuccess) {
     acd_dev_record_error(-1, "batch allocation");
     rc = -1;
   }
@@ -1033,13 +1081,15 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   if (!rc) {
     for (size_t i = 0; i < count; i++) {
       h_args[i] = make_args(lps[i], key);
-      h_args[i].ev = nullptr;
-      h_args[i].ev_cap = 0;
+      if (!logged) {
+        h_args[i].ev = nullptr;
+        h_args[i].ev_cap = 0;
+      }
       h_args[i].st = d_st + i;
       h_args[i].prof = nullptr;
       h_args[i].partials = nullptr;
       acd_solve_request qi = *q;
-      qi.log = 0;
+      if (!logged) qi.log = 0;
       init_state(&h_st[i], lps[i], &qi);
       h_slots[i] = (uint32_t)i;
       cls[i] = (uint8_t)size_class(lps[i]);
@@ -1060,10 +1110,13 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
             hipSuccess ||
         hipMemsetAsync(d_flags, 0, count * sizeof(uint32_t), S()) !=
             hipSuccess ||
+        hipMemsetAsync(d_ticks, 0, 2 * count * sizeof(uint64_t), S()) !=
+            hipSuccess ||
         hipStreamSynchronize(S()) != hipSuccess)
       rc = -1;
   }
-  const BatchBufs bufs = {d_args, d_st, d_count, d_done, d_bytes, d_flags};
+  const BatchBufs bufs = {d_args,  d_st,    d_count, d_done,
+                          d_bytes, d_flags, d_ticks};
   Group grp[kWpbChoices] = {};
   const bool graphs = use_graphs();
   // Relaunch over the unfinished LPs (compacted between rounds) until all
@@ -1092,6 +1145,10 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
       {
         float rms = 0;
         if (hipEventElapsedTime(&rms, r0, r1) == hipSuccess) kernel_ms += rms;
+      }
+      if (logged && drain_events(lps[0], d_st, sink, ctx)) {
+        rc = -1;
+        break;
       }
       for (int i = 0; i < kWpbChoices; i++)
         if (grp[i].n) launches += kRoundLaunches;
@@ -1170,6 +1227,17 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
                  hipMemcpyDeviceToHost) != hipSuccess ||
        hipMemcpy(&t0, d_t0, sizeof t0, hipMemcpyDeviceToHost) != hipSuccess))
     rc = -1;
+  uint64_t tick_ops = 0, tick_ctl = 0;
+  if (!rc) {
+    uint64_t* h_ticks = (uint64_t*)malloc(2 * count * sizeof(uint64_t));
+    if (h_ticks && hipMemcpy(h_ticks, d_ticks, 2 * count * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost) == hipSuccess)
+      for (size_t i = 0; i < count; i++) {
+        tick_ops += h_ticks[2 * i];
+        tick_ctl += h_ticks[2 * i + 1];
+      }
+    free(h_ticks);
+  }
   if (!rc) {
     int khz = 100000;
     hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate,
@@ -1202,6 +1270,8 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
         stats->waves_per_lp[cls[i]] = kWpbOf[lp_choice(cls[i], forced, c0)];
       stats->device_ms = ms;
       stats->kernel_ms = kernel_ms;
+      stats->lp_ops_ms = khz > 0 ? (double)tick_ops / khz : 0.0;
+      stats->lp_ctl_ms = khz > 0 ? (double)tick_ctl / khz : 0.0;
       stats->launches = launches;
       stats->inner_iterations = inner;
       stats->alg_bytes = (double)total_bytes;
@@ -1219,6 +1289,7 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   hipFree(d_bytes);
   hipFree(d_t0);
   hipFree(d_flags);
+  hipFree(d_ticks);
   free(h_args);
   free(h_st);
   free(h_slots);
@@ -1240,7 +1311,7 @@ struct acd_dev_pool {
   Args* d_args;
   State* d_st;
   uint32_t *d_slots, *d_count, *d_flags;
-  uint64_t *d_done, *d_bytes;
+  uint64_t *d_done, *d_bytes, *d_ticks;
   uint32_t* h_flags;  // pinned
   uint32_t* h_tmp;
   uint8_t* choice;    // per slot
@@ -1266,6 +1337,8 @@ struct acd_dev_pool* acd_dev_pool_create(uint32_t cap, int precision) {
       hipMalloc((void**)&p->d_flags, cap * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc((void**)&p->d_done, cap * sizeof(uint64_t)) != hipSuccess ||
       hipMalloc((void**)&p->d_bytes, cap * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc((void**)&p->d_ticks, 2 * cap * sizeof(uint64_t)) !=
+          hipSuccess ||
       hipHostMalloc((void**)&p->h_flags, cap * sizeof(uint32_t),
                     hipHostMallocDefault) != hipSuccess) {
     acd_dev_record_error(-1, "pool allocation");
@@ -1275,6 +1348,7 @@ struct acd_dev_pool* acd_dev_pool_create(uint32_t cap, int precision) {
   hipMemsetAsync(p->d_flags, 0, cap * sizeof(uint32_t), S());
   hipMemsetAsync(p->d_count, 0, sizeof(uint32_t), S());
   hipMemsetAsync(p->d_bytes, 0, cap * sizeof(uint64_t), S());
+  hipMemsetAsync(p->d_ticks, 0, 2 * cap * sizeof(uint64_t), S());
   hipStreamSynchronize(S());
   return p;
 }
@@ -1289,6 +1363,7 @@ void acd_dev_pool_free(struct acd_dev_pool* p) {
   hipFree(p->d_flags);
   hipFree(p->d_done);
   hipFree(p->d_bytes);
+  hipFree(p->d_ticks);
   if (p->h_flags) hipHostFree(p->h_flags);
   free(p->h_tmp);
   free(p->choice);
@@ -1341,8 +1416,8 @@ int acd_dev_pool_round(struct acd_dev_pool* p, const uint32_t* slots,
     if (slots[i] >= p->cap) return -1;
   if (n) {
     Group grp[kWpbChoices] = {};
-    const BatchBufs bufs = {p->d_args,  p->d_st,   p->d_count,
-                            p->d_done,  p->d_bytes, p->d_flags};
+    const BatchBufs bufs = {p->d_args,  p->d_st,    p->d_count, p->d_done,
+                            p->d_bytes, p->d_flags, p->d_ticks};
     if (upload_groups(slots, n, p->choice, p->h_tmp, p->d_slots, grp) ||
         run_round(p->key, grp, launches, budget, bufs, false, false)) {
       acd_dev_record_error(-1, "pool round");

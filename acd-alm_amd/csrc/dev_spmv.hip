@@ -49,15 +49,30 @@ __device__ __forceinline__ double row_dot(const double* __restrict__ v,
   return acc;
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs (block b runs on XCD
+// b mod 8).  With XCD = true, block b takes the tile of block
+// (b mod 8) * per + b / 8 (per = blocks per XCD; the grid is a multiple of
+// 8), so each XCD works on one contiguous eighth of the rows: where row
+// blocks touch their own column blocks (the multicommodity flow LP's
+// commodities), each XCD's gathers then stay within a slice of x that
+// its own 4 MB L2 holds.
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t tile_of_block(bool xcd) {
+  if (!xcd) return blockIdx.x;
+  const uint32_t per = gridDim.x / kXcds;
+  return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
 // R slices per wavefront, interleaved chunk by chunk so R independent
 // row sums are in flight per lane.
-template <int U, int R, bool NT>
+template <int U, int R, bool NT, bool XCD = false>
 __global__ void __launch_bounds__(64 * kWaves)
 k_spmv1(const uint64_t* __restrict__ off, const uint32_t* __restrict__ col,
         const double* __restrict__ val, uint32_t nrows, uint32_t nslices,
         const double* __restrict__ x, double* __restrict__ y) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t s0 = (blockIdx.x * kWaves + threadIdx.x / 64) * R;
+  const uint32_t s0 = (tile_of_block(XCD) * kWaves + threadIdx.x / 64) * R;
   if (R == 1) {
     if (s0 >= nslices) return;
     const uint64_t base = off[s0];
@@ -183,16 +198,19 @@ int check(const char* what) {
   return 0;
 }
 
-template <int U, int R, bool NT>
+template <int U, int R, bool NT, bool XCD = false>
 int launch1_t(const struct acd_sell* op, double* y, const double* x) {
   const uint32_t waves = (op->nslices + R - 1) / R;
-  const unsigned grid = (waves + kWaves - 1) / kWaves;
-  k_spmv1<U, R, NT><<<grid, 64 * kWaves, 0, S()>>>(op->off, op->col, op->val,
-                                                   op->nrows, op->nslices, x, y);
+  unsigned grid = (waves + kWaves - 1) / kWaves;
+  if (XCD) grid = (grid + kXcds - 1) / kXcds * kXcds;  // tiles past the
+                                                        // end return
+  k_spmv1<U, R, NT, XCD><<<grid, 64 * kWaves, 0, S()>>>(
+      op->off, op->col, op->val, op->nrows, op->nslices, x, y);
   return check("spmv launch");
 }
 
-// Variant table: (unroll, slices per wave, nontemporal loads).
+// Variant table: (unroll, slices per wave, nontemporal loads, XCD-contiguous
+// tiles).
 int launch1(const struct acd_sell* op, double* y, const double* x, int variant) {
   if (op->nslices == 0) return 0;
   switch (variant) {
@@ -205,6 +223,10 @@ int launch1(const struct acd_sell* op, double* y, const double* x, int variant) 
     case 6: return launch1_t<4, 4, true>(op, y, x);
     case 7: return launch1_t<16, 1, true>(op, y, x);
     case 8: return launch1_t<2, 4, true>(op, y, x);
+    case 9: return launch1_t<4, 1, false, true>(op, y, x);
+    case 10: return launch1_t<8, 1, false, true>(op, y, x);
+    case 11: return launch1_t<8, 1, true, true>(op, y, x);
+    case 12: return launch1_t<4, 2, true, true>(op, y, x);
     default: return launch1_t<4, 1, true>(op, y, x);
   }
 }

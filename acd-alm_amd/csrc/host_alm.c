@@ -122,10 +122,13 @@ int alm_solve_ex(alm_t * alm, size_t niter, double * x, size_t nvars,
         struct acd_log_ctx ctx = {log, 0};
         int small = m + n <= 4000;
         int rc;
+        /* The engine depends on the LP's size only, never on whether a
+         * log is written: both engines sum in their own fixed orders, so
+         * logging must not change which one runs. */
         if (o.engine == ACD_ENGINE_WORKGROUP
-            || (o.engine == ACD_ENGINE_AUTO && log == NULL && small)) {
-                q.log = 0;
-                rc = acd_dev_solve_batch(&dev, 1, &q, &r, NULL, NULL);
+            || (o.engine == ACD_ENGINE_AUTO && small)) {
+                rc = acd_dev_solve_batch(&dev, 1, &q, &r, NULL, NULL,
+                                         acd_log_sink, &ctx);
         } else {
                 rc = acd_dev_solve(dev, &q, &r, acd_log_sink, &ctx);
         }
@@ -204,7 +207,8 @@ int alm_solve_batch(size_t count, alm_t * const * alms, size_t niter,
         q.stop_kkt = o.stop_kkt;
         struct acd_batch_stats st;
         memset(&st, 0, sizeof st);
-        int rc = acd_dev_solve_batch(lps, count, &q, res, OUT_done_ms, &st);
+        int rc = acd_dev_solve_batch(lps, count, &q, res, OUT_done_ms, &st,
+                                     NULL, NULL);
         if (rc) acd_fail("alm_solve_batch: device solve failed");
         for (size_t i = 0; i < count; i++) {
                 /* A failed LP (non-finite iterate, positive model
@@ -224,6 +228,8 @@ int alm_solve_batch(size_t count, alm_t * const * alms, size_t niter,
                 OUT_info->inner_iterations = st.inner_iterations;
                 OUT_info->alg_bytes = st.alg_bytes;
                 OUT_info->kernel_ms = st.kernel_ms;
+                OUT_info->lp_ops_ms = st.lp_ops_ms;
+                OUT_info->lp_ctl_ms = st.lp_ctl_ms;
                 for (int k = 0; k < ACD_BATCH_CLASSES; k++)
                         OUT_info->waves_per_lp[k] = st.waves_per_lp[k];
         }

@@ -27,6 +27,8 @@ int acd_copy_to_host(void * dst, const void * src, size_t bytes)
 
 int acd_copy_device(void * dst, const void * src, size_t bytes)
 {
+        if (bytes % 16 == 0 && ((uintptr_t)dst | (uintptr_t)src) % 16 == 0)
+                return acd_dev_copy_kernel(dst, src, bytes) || acd_dev_sync();
         return acd_dev_d2d(dst, src, bytes) || acd_dev_sync();
 }
 

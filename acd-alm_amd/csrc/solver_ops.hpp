@@ -944,13 +944,24 @@ struct Ctl {
   }
 
   // Runs continuations until one issues an op (returns) or the solve ends.
+  // OutOfLine = true calls the continuation as a function of its own: the
+  // caller's register allocation then does not have to hold the
+  // continuation's temporaries beside its own (the batched kernel, whose
+  // occupancy the register count sets, uses it; the control kernel
+  // inlines it).
+  template <bool OutOfLine = false>
   __device__ void advance() {
     s.op = OP_NONE;
     for (int guard = 0; guard < 256 && s.op == OP_NONE && s.pc != PC_DONE;
          guard++) {
-      step();
+      if (OutOfLine)
+        step_call();
+      else
+        step();
     }
   }
+
+  __device__ __attribute__((noinline)) void step_call() { step(); }
 
   __device__ void step() {
     switch (s.pc) {

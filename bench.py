@@ -298,13 +298,15 @@ def run_batch(args, lib):
         log("warmup %d: %.2f s device, %.2f s wall" % (w, info.device_ms / 1e3,
                                                        wall))
     L.acd_synchronize()
-    dev_ms = ker_ms = alg = walls = 0.0
+    dev_ms = ker_ms = alg = walls = ops_ms = ctl_ms = 0.0
     launches = inner = 0
     dones = []
     for k in range(args.steps):
         out, done, info, wall = step()
         dev_ms += info.device_ms
         ker_ms += info.kernel_ms
+        ops_ms += info.lp_ops_ms
+        ctl_ms += info.lp_ctl_ms
         alg += info.alg_bytes
         launches += int(info.launches)
         inner += int(info.inner_iterations)
@@ -336,7 +338,13 @@ def run_batch(args, lib):
             "launches": launches,
             "avg_us": round(ker_ms * 1e3 / max(launches, 1), 1),
             "alg_bytes_per_launch": round(alg / max(launches, 1)),
-            "kernel_ms_per_step": ker_ms / args.steps},
+            "kernel_ms_per_step": ker_ms / args.steps,
+            "lp_time_split": {"passes": round(ops_ms / (ops_ms + ctl_ms), 4),
+                              "control": round(ctl_ms / (ops_ms + ctl_ms),
+                                               4)},
+            "lp_busy_ms_per_step": (ops_ms + ctl_ms) / args.steps},
+        "done_ms_deciles": [round(float(np.percentile(done, q)), 1)
+                            for q in range(0, 101, 10)],
         "setup_s": setup,
     }
     legs = {}
@@ -654,6 +662,9 @@ def main():
                     help="setcover/mcf: ALM outer-iteration cap")
     ap.add_argument("--log", action="store_true",
                     help="setcover/mcf: stream the ALM log to stderr")
+    ap.add_argument("--copy-check", type=int, default=0, metavar="BYTES",
+                    help="first run one plain device copy kernel of BYTES "
+                         "(the reference for calibrating byte counters)")
     ap.add_argument("--rate", type=float, default=2000.0,
                     help="serve: offered requests per second")
     ap.add_argument("--seconds", type=float, default=1.0,
@@ -670,6 +681,14 @@ def main():
 
     from acdalm import capi
     lib = capi.load()
+    if args.copy_check:
+        L = lib.lib
+        src = L.acd_device_alloc(args.copy_check)
+        dst = L.acd_device_alloc(args.copy_check)
+        if not src or not dst or L.acd_copy_device(dst, src, args.copy_check):
+            raise RuntimeError("copy check failed: %s" % (lib.last_error(),))
+        L.acd_device_free(src)
+        L.acd_device_free(dst)
     if args.workload == "serve":
         r = serve_leg(lib, args.rate, args.seconds, args.stop_kkt,
                       prec={"mixed": 1, "fp64": 0}[args.precision],

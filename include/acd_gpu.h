@@ -42,7 +42,8 @@ int acd_spmv_device(const sparse_matrix_t *, int transpose,
 int acd_spmv_device_time(const sparse_matrix_t *, int transpose,
                          double * d_y, const double * d_x, int reps,
                          double * OUT_ms_per_launch);
-/* SpMV kernel variants (unroll depth x slices per wavefront; all give
+/* SpMV kernel variants (unroll depth x slices per wavefront x load kind x
+ * XCD-contiguous row tiles; all give
  * bit-identical results).  Times variant `variant` like
  * acd_spmv_device_time; acd_spmv_set_variant makes one the default for
  * subsequent products. */
@@ -50,7 +51,7 @@ int acd_spmv_variant_time(const sparse_matrix_t *, int transpose,
                           double * d_y, const double * d_x, int reps,
                           int variant, double * OUT_ms_per_launch);
 int acd_spmv_set_variant(int variant);
-enum { ACD_SPMV_VARIANTS = 9 };
+enum { ACD_SPMV_VARIANTS = 13 };
 /* Autotuner: times every variant on both operators of the matrix (`reps`
  * products each) and keeps the fastest per operator for
  * sparse_matrix_multiply and acd_spmv_device.  Variants only schedule
@@ -162,6 +163,11 @@ struct acd_batch_info {
          * each round of launches on the library stream, summed, so the
          * host's work between rounds is excluded (device_ms includes it). */
         double kernel_ms;
+        /* Time the LPs' workgroups spent, summed over the LPs (LP-ms), in
+         * the data-parallel passes with their reductions and in the
+         * one-thread control step that takes the reference's decisions
+         * between passes. */
+        double lp_ops_ms, lp_ctl_ms;
 };
 /* alm_solve_ex on `count` independent ALM objects at once: one workgroup
  * per LP, every LP on the device together, repeated launches of 32 inner

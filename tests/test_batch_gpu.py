@@ -141,6 +141,30 @@ def test_single_small_lp_routes_to_the_workgroup_solver(gpu):
     assert same(r_auto[:4], r_wg[:4]) and same(r_auto[:4], alone[0])
 
 
+def test_writing_a_log_does_not_change_the_result(gpu):
+    """The engine follows the LP's size only: a small LP runs on the
+    one-workgroup solver with or without a log (the log is replayed from
+    its event ring), so logging leaves x, lambda and the diagnosis
+    unchanged; the log has one line per outer iteration in the
+    reference's format (alm.c:266-285)."""
+    import re
+    lp = I.batch(1, seed=15, mmin=60, mmax=80)[0]
+    m = gpu.make_matrix(lp, 0)
+    a = gpu.alm_make(m, lp)
+    print("[batch] log vs no log", flush=True)
+    quiet = gpu.alm_solve_ex(a, 1000, np.zeros(lp.ncols), np.zeros(lp.nrows))
+    gpu.lib.acd_alm_reset(a)
+    loud = gpu.alm_solve_ex(a, 1000, np.zeros(lp.ncols), np.zeros(lp.nrows),
+                            log=True)
+    gpu.alm_free(a)
+    gpu.free_matrix(m)
+    assert same(quiet[:4], loud[:4])
+    outer = [ln for ln in loud[4].splitlines() if re.match(r"^\s*\d+: ", ln)]
+    assert len(outer) >= 1
+    for ln in outer:
+        assert ln.rstrip().endswith("-stalled-") or re.search(r"\(\s*\S+\)$", ln.rstrip()), ln
+
+
 def test_result_does_not_depend_on_the_workgroup_size(gpu, monkeypatch):
     """Every op is reduced as 4 virtual waves, so workgroups of 1, 2 or 4
     wavefronts give bit-identical results."""

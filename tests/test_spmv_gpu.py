@@ -125,14 +125,14 @@ def test_autotuned_variants_give_identical_products(gpu, oracle):
     rng = np.random.default_rng(5)
     x = rng.standard_normal(lp.ncols)
     y = rng.standard_normal(lp.nrows)
-    for v in range(9):
+    for v in range(13):
         gpu.lib.acd_spmv_set_variant(v)
         assert bit_equal(gpu.multiply(mg, x, 0), oracle.multiply(mo, x, 0)), v
         assert bit_equal(gpu.multiply(mg, y, 1), oracle.multiply(mo, y, 1)), v
     gpu.lib.acd_spmv_set_variant(1)
     tuned = (ctypes.c_int * 2)()
     assert gpu.lib.acd_spmv_autotune(mg, 3, tuned) == 0
-    assert 0 <= tuned[0] < 9 and 0 <= tuned[1] < 9
+    assert 0 <= tuned[0] < 13 and 0 <= tuned[1] < 13
     assert bit_equal(gpu.multiply(mg, x, 0), oracle.multiply(mo, x, 0))
     assert bit_equal(gpu.multiply(mg, y, 1), oracle.multiply(mo, y, 1))
     gpu.free_matrix(mg)

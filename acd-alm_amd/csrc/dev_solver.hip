@@ -200,6 +200,71 @@ struct BlockG {
 // 344 against 336 solves/s; bounding it to 4 waves per SIMD spills and
 // ran 316).
 constexpr bool kBatchCtlCall = true;
+
+// Vector pairs that a batched LP may keep in LDS, in the order they are
+// given LDS (first the candidate point z / zp that the A pass gathers,
+// then the violation pairs the A' pass gathers and the momentum update
+// reads and writes): {side (-1: n-side, +1: m-side), role, partner role};
+// each pair is closed under the solver's pointer swaps (approx.c
+// :461-477 and the speculative commit).
+constexpr int kLdsPairs = 4;
+__device__ constexpr int kLdsPair[kLdsPairs][3] = {
+    {-1, NZ, NZP}, {1, MYV, MSY}, {1, MZV, MZPV}, {1, MXV, MSX}};
+// Per-workgroup LDS for resident vectors: 3 LP workgroups per CU (the
+// kernel's register-bound occupancy) fit in the 160 KB of LDS with their
+// static records (~1.5 KB each).
+// Measured on 4,096 LPs: 51 KB (3 per CU) 387 solves/s, 76 KB (2 per CU)
+// 365, 150 KB (1 per CU) 281; without residency 345.
+constexpr uint32_t kLdsBudget = 51u << 10;
+
+extern __shared__ __align__(16) unsigned char lds_vecs[];
+
+// Copies the resident slots between their global homes and LDS
+// (to_lds) or back; slot order as assigned by k_batch.
+template <int WPB>
+__device__ void lds_copy(const Args& la, const State& ls, uint32_t resident,
+                         void* const* home, bool to_lds) {
+  size_t off = 0;
+  uint32_t k = 0;
+  for (int q = 0; q < kLdsPairs; q++) {
+    if (!((resident >> q) & 1)) continue;
+    const bool nside = kLdsPair[q][0] < 0;
+    const uint32_t len = nside ? la.n : la.m;
+    const size_t bytes = (len * (size_t)la.si + 15) & ~(size_t)15;
+    for (int e = 0; e < 2; e++) {
+      // 4-byte words (iterates are 8 or 4 bytes)
+      uint32_t* g = reinterpret_cast<uint32_t*>(home[k]);
+      uint32_t* l = reinterpret_cast<uint32_t*>(lds_vecs + off);
+      const uint32_t words = (uint32_t)(len * (size_t)la.si / 4);
+      for (uint32_t i = threadIdx.x; i < words; i += 64 * WPB) {
+        if (to_lds)
+          l[i] = g[i];
+        else
+          g[i] = l[i];
+      }
+      off += bytes;
+      k++;
+    }
+  }
+}
+
+// LDS bytes and pair mask for an LP of m rows and n columns with
+// si-byte iterates under the budget: pairs are taken in order, each one
+// that still fits.
+uint32_t lds_plan(uint32_t m, uint32_t n, uint32_t si, uint32_t* bytes) {
+  const size_t bn = 2 * (((size_t)n * si + 15) & ~(size_t)15);
+  const size_t bm = 2 * (((size_t)m * si + 15) & ~(size_t)15);
+  const size_t need[kLdsPairs] = {bn, bm, bm, bm};
+  size_t used = 0;
+  uint32_t mask = 0;
+  for (int q = 0; q < kLdsPairs; q++) {
+    if (used + need[q] > kLdsBudget) continue;
+    used += need[q];
+    mask |= 1u << q;
+  }
+  *bytes = (uint32_t)used;
+  return mask;
+}
 template <class VI, class VC, class MT, int WPB>
 __global__ void __launch_bounds__(64 * WPB)
 __attribute__((amdgpu_waves_per_eu(3)))
@@ -224,6 +289,40 @@ k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
   }
   __syncthreads();
   if (ls.pc == PC_DONE || ls.error) return;
+  // Vector pairs kept in LDS for this launch (Args::resident): their
+  // roles' pointers are redirected to LDS copies, so gathers and epilogue
+  // accesses of those vectors cost LDS latency instead of L2 / last-level
+  // cache latency; the ops are unchanged (they address vectors through
+  // generic pointers).  Each pair is closed under the solver's pointer
+  // swaps, so at the end every role again points into one of the pair's
+  // LDS slots and is written back to that slot's global home.
+  __shared__ void* home[2 * kLdsPairs];
+  __shared__ uint32_t nslot;
+  const uint32_t resident = la.resident;
+  if (resident) {
+    if (tid == 0) {
+      uint32_t k = 0;
+      size_t off = 0;
+      for (int q = 0; q < kLdsPairs; q++) {
+        if (!((resident >> q) & 1)) continue;
+        const bool nside = kLdsPair[q][0] < 0;
+        const uint32_t len = nside ? la.n : la.m;
+        const size_t bytes = (len * (size_t)la.si + 15) & ~(size_t)15;
+        for (int e = 0; e < 2; e++) {
+          const int r = kLdsPair[q][1 + e];
+          double*& ptr = nside ? ls.pn[r] : ls.pm[r];
+          home[k] = ptr;
+          ptr = reinterpret_cast<double*>(lds_vecs + off);
+          off += bytes;
+          k++;
+        }
+      }
+      nslot = k;
+    }
+    __syncthreads();
+    lds_copy<WPB>(la, ls, resident, home, true);
+    __syncthreads();
+  }
   const uint64_t stop_at = ls.inner_total + budget;
   uint64_t moved = 0;
   // Wall-clock ticks of this launch spent in ops (passes + reductions)
@@ -271,6 +370,30 @@ k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
     if (ls.pc == PC_DONE || ls.error || ls.op == OP_NONE ||
         ls.inner_total >= stop_at)
       break;
+  }
+  if (resident) {
+    __syncthreads();
+    lds_copy<WPB>(la, ls, resident, home, false);
+    if (tid == 0) {  // roles back to the global homes of their slots
+      size_t off = 0;
+      uint32_t k = 0;
+      for (int q = 0; q < kLdsPairs; q++) {
+        if (!((resident >> q) & 1)) continue;
+        const bool nside = kLdsPair[q][0] < 0;
+        const uint32_t len = nside ? la.n : la.m;
+        const size_t bytes = (len * (size_t)la.si + 15) & ~(size_t)15;
+        double* s0 = reinterpret_cast<double*>(lds_vecs + off);
+        double* s1 = reinterpret_cast<double*>(lds_vecs + off + bytes);
+        for (int e = 0; e < 2; e++) {
+          const int r = kLdsPair[q][1 + e];
+          double*& ptr = nside ? ls.pn[r] : ls.pm[r];
+          ptr = (double*)(ptr == s0 ? home[k] : ptr == s1 ? home[k + 1] : ptr);
+        }
+        off += 2 * bytes;
+        k += 2;
+      }
+    }
+    __syncthreads();
   }
   {
     const uint64_t* ss = reinterpret_cast<const uint64_t*>(&ls);
@@ -628,29 +751,30 @@ struct BatchBufs {
 
 template <int WPB>
 void launch_batch_w(int key, uint32_t nblocks, const uint32_t* slots,
-                    uint64_t budget, const BatchBufs& b, hipStream_t st) {
+                    uint64_t budget, const BatchBufs& b, hipStream_t st,
+                    uint32_t shmem) {
   const dim3 grid(nblocks), block(64 * WPB);
 #define ACD_BATCH_ARGS \
   b.args, b.st, slots, budget, b.done, b.t_done, b.bytes, b.flags, b.ticks
   switch (key) {
     case K_MIX_F32:
-      k_batch<double, float, float, WPB><<<grid, block, 0, st>>>(
+      k_batch<double, float, float, WPB><<<grid, block, shmem, st>>>(
           ACD_BATCH_ARGS);
       break;
     case K_MIX_BF16:
-      k_batch<double, float, uint16_t, WPB><<<grid, block, 0, st>>>(
+      k_batch<double, float, uint16_t, WPB><<<grid, block, shmem, st>>>(
           ACD_BATCH_ARGS);
       break;
     case K_F32:
-      k_batch<float, float, float, WPB><<<grid, block, 0, st>>>(
+      k_batch<float, float, float, WPB><<<grid, block, shmem, st>>>(
           ACD_BATCH_ARGS);
       break;
     case K_F32_BF16:
-      k_batch<float, float, uint16_t, WPB><<<grid, block, 0, st>>>(
+      k_batch<float, float, uint16_t, WPB><<<grid, block, shmem, st>>>(
           ACD_BATCH_ARGS);
       break;
     default:
-      k_batch<double, double, double, WPB><<<grid, block, 0, st>>>(
+      k_batch<double, double, double, WPB><<<grid, block, shmem, st>>>(
           ACD_BATCH_ARGS);
       break;
   }
@@ -659,13 +783,14 @@ void launch_batch_w(int key, uint32_t nblocks, const uint32_t* slots,
 
 // Waves per LP workgroup: 1, 2 or 4 (the same results for each).
 void launch_batch(int key, int wpb, uint32_t nblocks, const uint32_t* slots,
-                  uint64_t budget, const BatchBufs& b, hipStream_t st) {
+                  uint64_t budget, const BatchBufs& b, hipStream_t st,
+                  uint32_t shmem) {
   if (wpb == 1)
-    launch_batch_w<1>(key, nblocks, slots, budget, b, st);
+    launch_batch_w<1>(key, nblocks, slots, budget, b, st, shmem);
   else if (wpb == 2)
-    launch_batch_w<2>(key, nblocks, slots, budget, b, st);
+    launch_batch_w<2>(key, nblocks, slots, budget, b, st, shmem);
   else
-    launch_batch_w<4>(key, nblocks, slots, budget, b, st);
+    launch_batch_w<4>(key, nblocks, slots, budget, b, st, shmem);
 }
 
 // ---- workgroup-size groups ------------------------------------------------
@@ -749,6 +874,7 @@ GroupStreams* group_streams() {
 // stream.  A group with a captured graph replays it instead.
 struct Group {
   uint32_t n;          // LPs in the group
+  uint32_t shmem;      // dynamic LDS of its launches (resident vectors)
   uint32_t* d_slots;   // its slot list (device)
   hipGraphExec_t exec; // captured `launches` launches, if any
   bool have_exec;
@@ -780,7 +906,7 @@ int run_round(int key, Group* grp, uint32_t launches, uint64_t budget,
         hipSuccess)
       return -1;
     for (uint32_t l = 0; l < launches; l++)
-      launch_batch(key, kWpbOf[i], g.n, g.d_slots, budget, b, st);
+      launch_batch(key, kWpbOf[i], g.n, g.d_slots, budget, b, st, g.shmem);
     if (hipStreamEndCapture(st, &graph) != hipSuccess) return -1;
     const hipError_t e =
         hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
@@ -799,7 +925,7 @@ int run_round(int key, Group* grp, uint32_t launches, uint64_t budget,
       if (hipGraphLaunch(g.exec, st) != hipSuccess) return -1;
     } else {
       for (uint32_t l = 0; l < launches; l++)
-        launch_batch(key, kWpbOf[i], g.n, g.d_slots, budget, b, st);
+        launch_batch(key, kWpbOf[i], g.n, g.d_slots, budget, b, st, g.shmem);
     }
     if (gs && (hipEventRecord(gs->join[i], st) != hipSuccess ||
                hipStreamWaitEvent(S(), gs->join[i], 0) != hipSuccess))
@@ -820,14 +946,20 @@ void drop_graphs(Group* grp) {
 // choice[i] (index into kWpbOf) and upload the lists, group 0 first, into
 // d_slots.  Order inside a group is kept.
 int upload_groups(const uint32_t* live, uint32_t n, const uint8_t* choice,
-                  uint32_t* h_tmp, uint32_t* d_slots, Group* grp) {
+                  uint32_t* h_tmp, uint32_t* d_slots, Group* grp,
+                  const uint32_t* lds_bytes = nullptr) {
   drop_graphs(grp);
   uint32_t k = 0;
   for (int i = 0; i < kWpbChoices; i++) {
     grp[i].d_slots = d_slots + k;
+    grp[i].shmem = 0;
     uint32_t c = 0;
     for (uint32_t j = 0; j < n; j++)
-      if (choice[live[j]] == i) h_tmp[k + c++] = live[j];
+      if (choice[live[j]] == i) {
+        h_tmp[k + c++] = live[j];
+        if (lds_bytes && lds_bytes[live[j]] > grp[i].shmem)
+          grp[i].shmem = lds_bytes[live[j]];
+      }
     grp[i].n = c;
     k += c;
   }
@@ -1040,6 +1172,7 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   uint64_t* h_bytes = (uint64_t*)calloc(count, sizeof(uint64_t));
   uint8_t* cls = (uint8_t*)malloc(count);
   uint8_t* choice = (uint8_t*)malloc(count);
+  uint32_t* lds_bytes = (uint32_t*)calloc(count, sizeof(uint32_t));
   Args* d_args = nullptr;
   State* d_st = nullptr;
   uint32_t *d_slots = nullptr, *d_count = nullptr;
@@ -1054,7 +1187,7 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   hipEventCreate(&r1);
   double kernel_ms = 0;
   if (!h_args || !h_st || !h_slots || !h_tmp || !h_done || !h_bytes ||
-      !cls || !choice ||
+      !cls || !choice || !lds_bytes ||
       hipMalloc((void**)&d_args, count * sizeof(Args)) != hipSuccess ||
       hipMalloc((void**)&d_st, count * sizeof(State)) != hipSuccess ||
       hipMalloc((void**)&d_slots, count * sizeof(uint32_t)) != hipSuccess ||
@@ -1072,6 +1205,8 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   // configuration the tuner picks on this batch's own rounds.
   const int forced = forced_choice();
   const bool tune = forced < 0 && count >= kTuneMinLps;
+  const char* lds_env = getenv("ACD_BATCH_LDS");
+  const bool lds_on = !(lds_env && lds_env[0] == '0');
   const bool tune_log = getenv("ACD_BATCH_TUNE_LOG") != nullptr;
   int cfg = tune ? 0 : g_config, first_cfg = -1;
   int trial = tune ? 0 : -1;  // configuration on trial, -1: none
@@ -1088,6 +1223,9 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
       h_args[i].st = d_st + i;
       h_args[i].prof = nullptr;
       h_args[i].partials = nullptr;
+      if (lds_on && q->mode == ACD_MODE_ALM)
+        h_args[i].resident = lds_plan(lps[i]->m, lps[i]->n, h_args[i].si,
+                                      &lds_bytes[i]);
       acd_solve_request qi = *q;
       if (!logged) qi.log = 0;
       init_state(&h_st[i], lps[i], &qi);
@@ -1125,7 +1263,8 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   uint32_t active = (uint32_t)count;
   uint32_t* live = h_slots;
   uint32_t done = 0;
-  if (!rc && upload_groups(live, active, choice, h_tmp, d_slots, grp))
+  if (!rc &&
+      upload_groups(live, active, choice, h_tmp, d_slots, grp, lds_bytes))
     rc = -1;
   if (!rc) {
     hipEventRecord(e0, S());
@@ -1205,7 +1344,8 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
           choice[lp] = (uint8_t)lp_choice(cls[lp], forced, cfg);
         }
         active = k;
-        if (upload_groups(live, active, choice, h_tmp, d_slots, grp)) {
+        if (upload_groups(live, active, choice, h_tmp, d_slots, grp,
+                          lds_bytes)) {
           rc = -1;
           break;
         }
@@ -1298,6 +1438,7 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   free(h_bytes);
   free(cls);
   free(choice);
+  free(lds_bytes);
   return rc;
 }
 

@@ -197,6 +197,7 @@ struct Args {
   uint32_t ev_cap;              // 0: events off (batched solves)
   uint64_t nnz;                 // nonzeros of A
   uint32_t si, sc, sv;          // bytes per iterate / constant / value
+  uint32_t resident;            // batched kernel: vector pairs held in LDS
 };
 static_assert(sizeof(Args) % 8 == 0, "Args is copied as 64-bit words");
 

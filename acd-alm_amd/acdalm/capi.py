@@ -49,6 +49,21 @@ class BatchInfo(C.Structure):
                 ("lp_ops_ms", C.c_double), ("lp_ctl_ms", C.c_double)]
 
 
+class LpData(C.Structure):
+    """struct acd_lp_data (include/acd_gpu.h)."""
+    _fields_ = [("nrows", C.c_size_t), ("ncols", C.c_size_t),
+                ("nnz", C.c_size_t),
+                ("rows", C.POINTER(C.c_uint32)),
+                ("cols", C.POINTER(C.c_uint32)),
+                ("values", C.POINTER(C.c_double)),
+                ("rhs", C.POINTER(C.c_double)),
+                ("linear", C.POINTER(C.c_double)),
+                ("lower", C.POINTER(C.c_double)),
+                ("upper", C.POINTER(C.c_double)),
+                ("lambda_lower", C.POINTER(C.c_double)),
+                ("lambda_upper", C.POINTER(C.c_double))]
+
+
 class MpsLP(C.Structure):
     """struct acd_mps_lp (include/acd_gpu.h)."""
     _fields_ = [("nrows", C.c_size_t), ("ncols", C.c_size_t),
@@ -222,6 +237,13 @@ class CLib:
             ("acd_server_step", C.c_int, [vp, C.POINTER(C.c_uint64), sz,
                                           C.POINTER(sz)]),
             ("acd_server_pending", sz, [vp]),
+            ("acd_server_submit_lp", C.c_int, [vp, C.POINTER(LpData), dp, dp,
+                                               dp, C.POINTER(C.c_int),
+                                               C.c_uint64]),
+            ("acd_server_footprint", sz, [sz, sz, sz]),
+            ("acd_server_bytes_admitted", sz, [vp]),
+            ("acd_device_memory_info", C.c_int, [C.POINTER(sz),
+                                                 C.POINTER(sz)]),
             ("alm_solve_batch", C.c_int, [sz, C.POINTER(vp), sz,
                                           C.POINTER(dp), C.POINTER(dp),
                                           C.POINTER(dp), C.POINTER(C.c_int),
@@ -499,6 +521,13 @@ class CLib:
         return st
 
     # ---- GPU extras ----
+    def memory_info(self):
+        """(free, total) device bytes."""
+        f, t = sz(), sz()
+        if self.lib.acd_device_memory_info(C.byref(f), C.byref(t)):
+            raise RuntimeError("acd_device_memory_info failed")
+        return f.value, t.value
+
     def last_error(self):
         buf = C.create_string_buffer(256)
         code = self.lib.acd_last_error(buf, 256)
@@ -542,9 +571,32 @@ class Server:
                                % (self.lib.last_error(),))
         out = []
         for i in range(min(n.value, self.cap)):
-            x, lam, diag, ret = self.live.pop(done[i])
+            x, lam, diag, ret = self.live.pop(done[i])[:4]
             out.append((int(done[i]), ret.value, x, lam, diag))
         return out
+
+    def submit_lp(self, tag, lp, x0, lam0):
+        """Queue an LP given as host arrays; the library builds it on the
+        device when it is admitted and frees it when it completes."""
+        x = f64(x0).copy()
+        lam = f64(lam0).copy()
+        diag = np.full(3, np.nan)
+        ret = C.c_int(-1)
+        keep = [np.ascontiguousarray(lp.rows, np.uint32),
+                np.ascontiguousarray(lp.cols, np.uint32)] + \
+            [f64(a) for a in (lp.vals, lp.rhs, lp.linear, lp.lower, lp.upper,
+                              lp.lambda_lower, lp.lambda_upper)]
+        d = LpData(lp.nrows, lp.ncols, lp.nnz, _u(keep[0]), _u(keep[1]),
+                   *[_d(a) for a in keep[2:]])
+        self.live[tag] = (x, lam, diag, ret, keep, d)
+        rc = self.lib.lib.acd_server_submit_lp(self.s, C.byref(d), _d(x),
+                                               _d(lam), _d(diag),
+                                               C.byref(ret), tag)
+        if rc:
+            raise RuntimeError("acd_server_submit_lp failed")
+
+    def bytes_admitted(self):
+        return self.lib.lib.acd_server_bytes_admitted(self.s)
 
     def pending(self):
         return self.lib.lib.acd_server_pending(self.s)

@@ -29,8 +29,12 @@ int acd_dev_d2d(void * dst, const void * src, size_t bytes);
 int acd_dev_copy_kernel(void * dst, const void * src, size_t bytes);
 int acd_dev_memset(void * dst, int byte, size_t bytes);
 int acd_dev_sync(void);
+int acd_dev_mem_info(size_t * free_bytes, size_t * total_bytes);
 void * acd_dev_stream(void);            /* hipStream_t */
 int acd_dev_set_stream(void * stream);
+/* An extra stream (default priority, non-blocking) and its release. */
+void * acd_dev_stream_create(void);
+void acd_dev_stream_destroy(void * stream);
 int acd_dev_cu_count(void);
 /* First HIP error seen by the library since the last call (0 = none);
  * text copied into buf. */

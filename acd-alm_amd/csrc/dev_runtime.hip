@@ -64,6 +64,19 @@ int acd_dev_cu_count(void) { return g_cus; }
 
 void* acd_dev_stream(void) { return (void*)g_stream; }
 
+void* acd_dev_stream_create(void) {
+  if (acd_dev_init()) return nullptr;
+  hipStream_t s = nullptr;
+  if (note(hipStreamCreateWithFlags(&s, hipStreamNonBlocking),
+           "hipStreamCreate"))
+    return nullptr;
+  return (void*)s;
+}
+
+void acd_dev_stream_destroy(void* s) {
+  if (s) hipStreamDestroy((hipStream_t)s);
+}
+
 int acd_dev_set_stream(void* s) {
   g_stream = (hipStream_t)s;
   return 0;
@@ -141,6 +154,15 @@ int acd_dev_d2d(void* dst, const void* src, size_t bytes) {
 int acd_dev_memset(void* dst, int byte, size_t bytes) {
   if (bytes == 0) return 0;
   return note(hipMemsetAsync(dst, byte, bytes, g_stream), "hipMemsetAsync");
+}
+
+int acd_dev_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  if (acd_dev_init()) return -1;
+  size_t f = 0, t = 0;
+  if (note(hipMemGetInfo(&f, &t), "hipMemGetInfo")) return -1;
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return 0;
 }
 
 int acd_dev_sync(void) {

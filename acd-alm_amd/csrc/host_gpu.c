@@ -12,6 +12,11 @@ int acd_last_error(char * buf, size_t len)
 
 int acd_synchronize(void) { return acd_dev_sync(); }
 
+int acd_device_memory_info(size_t * OUT_free, size_t * OUT_total)
+{
+        return acd_dev_mem_info(OUT_free, OUT_total);
+}
+
 void * acd_device_alloc(size_t bytes) { return acd_dev_malloc(bytes); }
 void acd_device_free(void * p) { acd_dev_free(p); }
 

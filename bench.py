@@ -549,10 +549,12 @@ def serve_leg(lib, rate, seconds, stop_kkt, prec=1, seed=9,
               large_frac=0.01, budget_gb=64.0, cpu=None):
     """Request loop (continuous batching): Poisson arrivals at `rate` per
     second for `seconds`; 99% LPs built like the batched workload, 1% large
-    (m=100,000, n=300,000, 8 nonzeros per column).  Each request's ALM
-    object is made before the clock starts (its data device-resident); the
-    latency of a request runs from its arrival time to the step that
-    retires it."""
+    (m=100,000, n=300,000, 8 nonzeros per column).  Each request is
+    submitted as host arrays at its arrival time (acd_server_submit_lp):
+    the library builds its matrix and ALM object on the device when it is
+    admitted (by estimated footprint against the byte budget) and frees
+    them when it retires, so a request's latency, from its arrival to the
+    step that retires it, includes its device preprocessing."""
     from acdalm import instances, kkt
     rng = np.random.default_rng(seed)
     t, arrivals = 0.0, []
@@ -569,23 +571,23 @@ def serve_leg(lib, rate, seconds, stop_kkt, prec=1, seed=9,
     lps = [instances.random_lp(100000, 300000, 8, int(sd)) if lg else
            instances.random_lp(int(m), int(n), 5, int(sd))
            for lg, m, n, sd in zip(large, ms, ns, seeds)]
-    mats = [lib.make_matrix(lp, 0) for lp in lps]
-    alms = [lib.alm_make(m, lp) for m, lp in zip(mats, lps)]
-    setup = time.perf_counter() - t0
+    gen_s = time.perf_counter() - t0
     dev_kkt = stop_kkt * (0.9 if prec else 1.0)
     srv = lib.server(int(budget_gb * 2**30), slots=2048, large_rows=50000,
                      precision=prec, stop_kkt=dev_kkt)
     lat = np.zeros(len(lps))
     results = {}
     nxt = 0
+    peak = 0
     start = time.perf_counter()
     while nxt < len(lps) or srv.pending():
         now = time.perf_counter() - start
         while nxt < len(lps) and arrivals[nxt] <= now:
-            srv.submit(nxt, alms[nxt], np.zeros(lps[nxt].ncols),
-                       np.zeros(lps[nxt].nrows))
+            srv.submit_lp(nxt, lps[nxt], np.zeros(lps[nxt].ncols),
+                          np.zeros(lps[nxt].nrows))
             nxt += 1
         done = srv.step()
+        peak = max(peak, srv.bytes_admitted())
         now = time.perf_counter() - start
         for tag, ret, x, lam, diag in done:
             lat[tag] = (now - arrivals[tag]) * 1e3
@@ -602,23 +604,22 @@ def serve_leg(lib, rate, seconds, stop_kkt, prec=1, seed=9,
             large_inner = 0
             if large.any():
                 i = int(np.flatnonzero(large)[0])
-                lib.lib.acd_alm_reset(alms[i])
-                lib.alm_solve_ex(alms[i], 1000, np.zeros(lps[i].ncols),
+                m = lib.make_matrix(lps[i], 0)
+                a = lib.alm_make(m, lps[i])
+                lib.alm_solve_ex(a, 1000, np.zeros(lps[i].ncols),
                                  np.zeros(lps[i].nrows), precision=prec,
                                  stop_kkt=dev_kkt)
                 large_inner = int(lib.stats().inner_iterations)
+                lib.alm_free(a)
+                lib.free_matrix(m)
             cpu_b = serve_cpu_baseline(lps, large, arrivals, large_inner, cpu)
         except Exception as e:  # reported, never silently replaced
             cpu_b = {"value": None, "error": str(e)[-300:]}
-    for a in alms:
-        lib.alm_free(a)
-    for m in mats:
-        lib.free_matrix(m)
     small = ~large
     return {
         "workload": "Poisson arrivals at %g/s for %g s: %d requests (%d "
-                    "large m=100000 n=300000)" % (rate, seconds, len(lps),
-                                                   int(large.sum())),
+                    "large m=100000 n=300000), submitted as host data"
+                    % (rate, seconds, len(lps), int(large.sum())),
         "value": len(lps) / end, "ms_per_step": end * 1e3,
         "offered_per_s": rate, "makespan_s": end,
         "p50_ms": float(np.percentile(lat, 50)),
@@ -628,7 +629,8 @@ def serve_leg(lib, rate, seconds, stop_kkt, prec=1, seed=9,
         "large_p50_ms": float(np.percentile(lat[large], 50)) if large.any() else None,
         "kkt_max": float(max(kk)), "kkt_ok": bool(max(kk) <= KKT_TOL),
         "all_converged": all(r[0] == 0 for r in results.values()),
-        "roofline": None, "cpu_baseline": cpu_b, "setup_s": setup,
+        "peak_admitted_bytes": int(peak), "budget_bytes": int(budget_gb * 2**30),
+        "roofline": None, "cpu_baseline": cpu_b, "generate_s": gen_s,
     }
 
 
@@ -669,6 +671,8 @@ def main():
                     help="serve: offered requests per second")
     ap.add_argument("--seconds", type=float, default=1.0,
                     help="serve: length of the arrival window")
+    ap.add_argument("--budget-gb", type=float, default=64.0,
+                    help="serve: device byte budget of the request loop")
     args = ap.parse_args()
     args.skip = set(filter(None, args.skip.split(",")))
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -692,6 +696,7 @@ def main():
     if args.workload == "serve":
         r = serve_leg(lib, args.rate, args.seconds, args.stop_kkt,
                       prec={"mixed": 1, "fp64": 0}[args.precision],
+                      budget_gb=args.budget_gb,
                       cpu=None if args.no_cpu else host_cpu())
         steps, warmup = 1, 0
     elif args.workload == "batch":

@@ -25,6 +25,8 @@ int acd_set_device(int device);
 int acd_last_error(char * buf, size_t len);
 /* Wait for all work queued by the library. */
 int acd_synchronize(void);
+/* Free and total bytes of the library's device (hipMemGetInfo). */
+int acd_device_memory_info(size_t * OUT_free, size_t * OUT_total);
 
 /* ---- device-resident data ---- */
 void * acd_device_alloc(size_t bytes);          /* zero-filled */
@@ -141,7 +143,9 @@ int alm_solve_ex(alm_t *, size_t niter, double * x, size_t nvars,
  * reference's asserts do). */
 enum acd_solve_error {
         ACD_SOLVE_UNBOUNDED = 1,   /* an iterate is not finite */
-        ACD_SOLVE_ESTIMATE = 2     /* a long step's model estimate > 0 */
+        ACD_SOLVE_ESTIMATE = 2,    /* a long step's model estimate > 0 */
+        ACD_SOLVE_NO_MEMORY = 3    /* a served LP given as host data could
+                                      not be built on the device */
 };
 #define ACD_BATCH_CLASSES 8
 struct acd_batch_info {
@@ -240,6 +244,34 @@ int acd_server_submit(acd_server_t *, alm_t * alm, double * x,
  * never drops one) and their count to *OUT_ndone. */
 int acd_server_step(acd_server_t *, uint64_t * OUT_done, size_t cap,
                     size_t * OUT_ndone);
+/* An LP as host arrays, in alm_make's conventions: A is nrows x ncols
+ * with nnz (row, column, value) triplets, no repeated pair; linear, lower,
+ * upper, lambda_lower and lambda_upper may be NULL (0, -inf, +inf, -inf,
+ * +inf).  The arrays must stay valid until the request completes. */
+struct acd_lp_data {
+        size_t nrows, ncols, nnz;
+        const uint32_t * rows, * cols;
+        const double * values;
+        const double * rhs, * linear, * lower, * upper;
+        const double * lambda_lower, * lambda_upper;
+};
+/* Queue one solve of an LP given as host data.  Nothing is allocated on
+ * the device until the request is admitted: then its matrix and ALM object
+ * are built there (sparse_matrix_make, unpermuted, and alm_make), and they
+ * are freed when it completes, so the device holds only the admitted
+ * requests (their estimated footprints summed stay within the budget,
+ * except that one request larger than the budget runs alone).  A request
+ * whose LP cannot be built completes with -ACD_SOLVE_NO_MEMORY.  The
+ * latency of a request therefore includes its device preprocessing. */
+int acd_server_submit_lp(acd_server_t *, const struct acd_lp_data * lp,
+                         double * x, double * lambda, double * OUT_diagnosis,
+                         int * OUT_ret, uint64_t tag);
+/* The footprint estimate admission uses for an LP of m rows, n columns
+ * and nnz entries: an upper estimate of its device bytes, in whole 2 MiB
+ * blocks of the device allocator plus one. */
+size_t acd_server_footprint(size_t m, size_t n, size_t nnz);
+/* Sum of the footprint estimates of the requests admitted now. */
+size_t acd_server_bytes_admitted(const acd_server_t *);
 /* Requests queued or running. */
 size_t acd_server_pending(const acd_server_t *);
 

@@ -89,3 +89,61 @@ def test_a_failing_request_completes_with_an_error(gpu):
     out = serve(gpu, lps + [bad], precision=0)
     assert out[3][0] == -1
     assert all(out[i][0] == 0 for i in range(3))
+
+
+def serve_host_data(gpu, lps, budget, waves=((0, None),), **opts):
+    """Like serve(), with the LPs submitted as host data: each is built on
+    the device at admission and freed at completion.  Returns the results
+    and the peak device bytes in use above the level after the server was
+    made (sampled after every step)."""
+    # the device allocator's first block and the library's one-time
+    # buffers are not the requests': made and freed before the baseline
+    m = gpu.make_matrix(lps[0], 0)
+    a = gpu.alm_make(m, lps[0])
+    gpu.alm_free(a)
+    gpu.free_matrix(m)
+    srv = gpu.server(budget, slots=64, large_rows=LARGE_ROWS, **opts)
+    free0, _ = gpu.memory_info()
+    out, rnd, peak = {}, 0, 0
+    todo = list(waves)
+    t0 = time.time()
+    while todo or srv.pending():
+        while todo and todo[0][0] <= rnd:
+            _, idx = todo.pop(0)
+            for i in (range(len(lps)) if idx is None else idx):
+                srv.submit_lp(i, lps[i], np.zeros(lps[i].ncols),
+                              np.zeros(lps[i].nrows))
+        for tag, ret, x, lam, diag in srv.step():
+            out[tag] = (ret, x, lam, diag)
+        free, _ = gpu.memory_info()
+        peak = max(peak, free0 - free)
+        rnd += 1
+        assert time.time() - t0 < 120, "request loop did not drain"
+    srv.close()
+    return out, peak
+
+
+def test_host_data_requests_are_admitted_by_footprint(gpu):
+    """Requests given as host arrays hold device memory only while admitted:
+    with a budget of a quarter of their summed footprint estimates, the
+    device memory in use stays within the budget, and every result equals
+    the result of the same LP served alone."""
+    lps = I.batch(24, seed=16, mmin=150, mmax=900)
+    est = [int(gpu.lib.acd_server_footprint(lp.nrows, lp.ncols, lp.nnz))
+           + (8 << 20 if lp.nrows >= LARGE_ROWS else 0) for lp in lps]
+    budget = max(sum(est) // 4, max(est))
+    print(f"[serve] host data: {len(lps)} LPs, estimates {sum(est)} B, "
+          f"budget {budget} B", flush=True)
+    waves = [(0, list(range(12))), (3, list(range(12, 24)))]
+    mixed, peak = serve_host_data(gpu, lps, budget, waves=waves, precision=1,
+                                  stop_kkt=0.9e-6)
+    print(f"[serve] peak device bytes in use {peak}", flush=True)
+    assert sorted(mixed) == list(range(len(lps)))
+    assert peak <= budget, (peak, budget)
+    for i in (0, 7, 13, 23):
+        alone, _ = serve_host_data(gpu, [lps[i]], 1 << 40, precision=1,
+                                   stop_kkt=0.9e-6)
+        assert same(mixed[i], alone[0]), i
+    for i, lp in enumerate(lps):
+        assert mixed[i][0] == 0
+        assert kkt(lp, mixed[i][1], mixed[i][2])["kkt"] <= 1e-6

@@ -251,6 +251,9 @@ class CLib:
                                           C.POINTER(C.c_uint64),
                                           C.POINTER(BatchInfo)]),
             ("acd_alm_reset", C.c_int, [vp]),
+            ("acd_cd_solve_parallel", C.c_int, [dp, sz, vp, sz, C.c_double,
+                                                C.c_double, C.c_double,
+                                                C.c_double, C.c_uint64, dp]),
             ("acd_spmv_variant_time", C.c_int, [vp, C.c_int, vp, vp, C.c_int,
                                                 C.c_int, C.POINTER(C.c_double)]),
             ("acd_spmv_set_variant", C.c_int, [C.c_int]),
@@ -378,6 +381,16 @@ class CLib:
                  min_delta=0.0, log=False, period=0, offset=0.0, pool=None):
         return self._inner(self.lib.cd_solve, a, x0, niter, max_pg,
                            max_value, min_delta, log, period, offset, pool)
+
+    def cd_solve_parallel(self, a, x0, niter, sample=1 / 16., seed=1,
+                          max_pg=0.0, max_value=-np.inf, offset=0.0):
+        """acd_cd_solve_parallel: (return code, x, diagnosis[5])."""
+        x = f64(x0).copy()
+        diag = np.full(5, np.nan)
+        rc = self.lib.acd_cd_solve_parallel(_d(x), x.shape[0], a, niter,
+                                            max_pg, max_value, offset,
+                                            sample, seed, _d(diag))
+        return rc, x, diag
 
     # ---- ALM ----
     def alm_make(self, m, lp):

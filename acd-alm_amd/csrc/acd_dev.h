@@ -152,7 +152,8 @@ enum acd_vec {
         ACD_V_COUNT
 };
 
-enum acd_mode { ACD_MODE_APPROX = 1, ACD_MODE_MPG = 2, ACD_MODE_ALM = 3 };
+enum acd_mode { ACD_MODE_APPROX = 1, ACD_MODE_MPG = 2, ACD_MODE_ALM = 3,
+                ACD_MODE_PCD = 4 };
 /* Working-copy precision (same values as acd_gpu.h's enum acd_precision). */
 enum { ACD_DEV_FP64 = 0, ACD_DEV_MIXED = 1, ACD_DEV_FP32 = 2 };
 
@@ -166,6 +167,11 @@ struct acd_solve_request {
         double stop_violation, stop_pg, precision_floor, inner_min_delta;
         int precision;            /* ACD_DEV_FP64 / _MIXED / _FP32 */
         double stop_kkt;          /* > 0: KKT stopping target (ALM) */
+        /* PCD only: sampling probability, seed, and steps per epoch
+         * (niter counts epochs) */
+        double pcd_p;
+        uint64_t pcd_seed;
+        uint32_t pcd_steps;
 };
 
 struct acd_solve_result {

@@ -595,6 +595,15 @@ void init_state(State* s, const acd_dev_lp* lp, const acd_solve_request* q) {
   switch (q->mode) {
     case ACD_MODE_APPROX: s->pc = PC_A_INIT; break;
     case ACD_MODE_MPG: s->pc = PC_M0; break;
+    case ACD_MODE_PCD: {
+      s->pc = PC_P0;
+      const double pr = q->pcd_p > 1 ? 1.0 : q->pcd_p;
+      s->pcd_thresh = pr >= 1 ? 0xffffffffu : (uint32_t)(pr * 4294967296.0);
+      s->pcd_steps = q->pcd_steps ? q->pcd_steps : 1;
+      s->pcd_seed = q->pcd_seed;
+      s->pcd_p = pr;
+      break;
+    }
     case ACD_MODE_ALM:
       s->pc = PC_L0;
       s->outer_niter = q->niter;

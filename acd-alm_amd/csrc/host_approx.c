@@ -264,6 +264,46 @@ int mpg_solve(double * x, size_t n, approx_t * a, size_t niter,
         return 2;
 }
 
+/* Parallel randomised coordinate descent with ESO steps (no counterpart
+ * in the reference, whose cd_solve (approx/cd.c:332-385) takes one
+ * coordinate at a time): each step updates a random sample of the
+ * coordinates, each with probability `sample`, by
+ * x_j <- clamp(x_j - g_j / v'_j) with v'_j = sum_i w_i a_ij^2
+ * (1 + sample (omega_i - 1)), omega_i the nonzeros of row i (the expected
+ * separable overapproximation of Richtarik and Takac for independent
+ * sampling, with per-row factors).  An epoch is ceil(1 / sample) steps;
+ * niter counts epochs; the stopping tests and diagnosis are cd_solve's. */
+int acd_cd_solve_parallel(double * x, size_t n, approx_t * a, size_t niter,
+                          double max_pg, double max_value, double offset,
+                          double sample, uint64_t seed,
+                          double * OUT_diagnosis)
+{
+        ACD_REQUIRE(n == a->nvars, "solve: n does not match the instance");
+        if (!(sample > 0) || sample > 1) return -1;
+        double * dx = acd_dev_lp_vec(a->dev, ACD_V_X);
+        if (acd_dev_h2d(dx, x, n * sizeof(double)))
+                acd_fail("acd_cd_solve_parallel");
+        struct acd_solve_request q;
+        memset(&q, 0, sizeof q);
+        q.mode = ACD_MODE_PCD;
+        q.niter = niter;
+        q.max_pg = max_pg;
+        q.max_value = max_value;
+        q.offset = offset;
+        q.pcd_p = sample;
+        q.pcd_seed = seed;
+        q.pcd_steps = (uint32_t)ceil(1.0 / sample);
+        struct acd_solve_result r;
+        int rc = acd_dev_solve(a->dev, &q, &r, NULL, NULL);
+        acd_check_result(&r, rc, "acd_cd_solve_parallel");
+        acd_record_stats(&r);
+        if (acd_dev_d2h(x, dx, n * sizeof(double)))
+                acd_fail("acd_cd_solve_parallel");
+        if (OUT_diagnosis != NULL)
+                for (int i = 0; i < 5; i++) OUT_diagnosis[i] = r.diag[i];
+        return r.ret;
+}
+
 /* ---- randomised coordinate descent (approx/cd.c) ---- */
 
 /* approx/cd.c:27-40: the column list shuffled with the process's random(),

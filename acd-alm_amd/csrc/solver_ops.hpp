@@ -66,7 +66,9 @@ enum Op : int32_t {
   OP_ALM_VIOL,
   OP_ARGMAX,
   OP_LAM_W,
-  OP_KKT
+  OP_KKT,
+  OP_PCD_STEP,
+  OP_PCD_SIZES
 };
 
 enum OpFlag : uint32_t {
@@ -101,8 +103,23 @@ enum Pc : int32_t {
   PC_L0, PC_L1, PC_LB0, PC_LB1, PC_LB2, PC_LB3,
   PC_LE0, PC_LE1, PC_LE2, PC_LE3, PC_LE4, PC_LE5,
   // approx_update
-  PC_U0, PC_U1, PC_U2
+  PC_U0, PC_U1, PC_U2,
+  // parallel coordinate descent
+  PC_P0, PC_PZ, PC_P1, PC_P2, PC_PI, PC_PS, PC_PV, PC_PV2, PC_PE, PC_PF,
+  PC_PF2
 };
+
+// Whether coordinate j is in the sample keyed by `key` (a SplitMix64-style
+// finaliser of key + j times the golden-ratio increment): the high 32 bits
+// below thresh, i.e. with probability thresh / 2^32.
+__device__ __forceinline__ bool pcd_pick(uint64_t key, uint32_t j,
+                                         uint32_t thresh) {
+  uint64_t h = key + 0x9E3779B97F4A7C15ull * (uint64_t)(j + 1);
+  h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+  h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+  h ^= h >> 31;
+  return (uint32_t)(h >> 32) < thresh;
+}
 
 struct Sell {
   const uint64_t* off;
@@ -161,6 +178,11 @@ struct State {
   double alm_diag[3];
   int32_t alm_ret, inner_reason, upd, bump;
   double scale, argmax;
+  // parallel coordinate descent: sampling threshold (of 2^32), steps per
+  // epoch, seed, sampling probability
+  uint32_t pcd_thresh, pcd_steps;
+  uint64_t pcd_seed;
+  double pcd_p;
   // bookkeeping
   uint64_t inner_total, events, ops;
   uint64_t t_last;              // wall clock when the last op was issued
@@ -824,6 +846,60 @@ struct Ops {
         });
         break;
       }
+      case OP_PCD_STEP: {  // parallel coordinate descent, one sampled step
+        // Each coordinate j is in this step's sample with probability
+        // thresh / 2^32, drawn from a hash of (seed, epoch, step, j), so the
+        // sample does not depend on the launch shape.  Sampled coordinates
+        // move by x_j <- clamp(x_j - g_j / v'_j) (v'_j from OP_PCD_SIZES,
+        // its inverse kept in z) with g = A'(w (A x - b)) + c at the step's
+        // start; unsampled rows of A' issue no loads.
+        VI* X = VN(NX);
+        const VI* ist = VN(NZ);
+        const VI* sv = VM(MXV);
+        const Sell& M = a.AT;
+        const uint64_t key = s.pcd_seed ^ (s.i << 32) ^ s.k;
+        for (uint32_t sl = g.gwave; sl < M.nslices; sl += g.nwaves) {
+          const uint32_t j = sl * 64 + g.lane;
+          if (j >= M.nrows || !pcd_pick(key, j, s.pcd_thresh)) continue;
+          const uint64_t base = M.off[sl];
+          const uint32_t len = (uint32_t)((M.off[sl + 1] - base) >> 6);
+          double acc = 0.0;
+          for (uint32_t k = 0; k < len; k++) {
+            const uint64_t e = base + (uint64_t)k * 64 + g.lane;
+            acc = acc + mval<MT>(M, e) * (double)sv[M.col[e]];
+          }
+          const double gj = acc + (double)pc[j];
+          X[j] = (VI)clamp_lu(j, (double)X[j] - gj * (double)ist[j]);
+        }
+        break;
+      }
+      case OP_PCD_SIZES: {  // ESO curvatures of the sampled steps
+        // v'_j = sum_i w_i a_ij^2 (1 + p (omega_i - 1)), omega_i the
+        // nonzeros in row i: E over the sample S of the quadratic term at
+        // x + h_S is at most p/2 sum_j v'_j h_j^2 (Cauchy-Schwarz over each
+        // row's sampled entries).  Its inverse goes to z, which the solve
+        // does not otherwise use; zero curvature steps like approx_update.
+        VI* ist = VN(NZ);
+        const double pr = s.pcd_p;
+        const Sell& M = a.AT;
+        for (uint32_t sl = g.gwave; sl < M.nslices; sl += g.nwaves) {
+          const uint64_t base = M.off[sl];
+          const uint32_t len = (uint32_t)((M.off[sl + 1] - base) >> 6);
+          double acc = 0.0;
+          for (uint32_t k = 0; k < len; k++) {
+            const uint64_t e = base + (uint64_t)k * 64 + g.lane;
+            const uint32_t i = M.col[e];
+            const double val = mval<MT>(M, e);
+            const double wi =
+                (double)a.p_w[i] * (1.0 + pr * ((double)a.beta[i] - 1.0));
+            acc = acc + (wi * val) * val;
+          }
+          const uint32_t j = sl * 64 + g.lane;
+          if (j < M.nrows)
+            ist[j] = (VI)((acc == 0 && (double)pc[j] == 0) ? 1.0 : 1.0 / acc);
+        }
+        break;
+      }
       default:
         break;
     }
@@ -888,6 +964,8 @@ __device__ inline double alg_bytes(int32_t op, uint32_t fl, const Args& a) {
     case OP_ARGMAX: return 8 * m;
     case OP_LAM_W: return 48 * m + ((fl & F_UPD) ? 8 * m : 0);
     case OP_KKT: return spT + 8 * m + 32 * n + 16 * m;
+    case OP_PCD_STEP: return spT + m * si + n * (2 * si + 4 * sc);
+    case OP_PCD_SIZES: return spT + m * 12 + n * (si + sc);
     default: return 0;
   }
 }
@@ -1436,6 +1514,85 @@ struct Ctl {
         }
         break;
       }
+      // ------------------------------------------- parallel coordinate descent
+      // Epochs of pcd_steps sampled steps each; after every step the
+      // violation and value at x are recomputed; after every epoch the
+      // projected gradient is, and the stopping tests of approx/cd.c:
+      // 353-366 apply (value < max_value: 1, pg < max_pg: 2).
+      case PC_P0:
+        issue(OP_INIT_X, 0, PC_PZ);
+        break;
+      case PC_PZ:
+        issue(OP_PCD_SIZES, 0, PC_P1);
+        break;
+      case PC_P1:
+        s.op_r0 = NX;
+        s.op_r1 = MXV;
+        issue(OP_VIOL, F_CX, PC_P2);
+        break;
+      case PC_P2:
+        s.quad_x = .5 * res[0];
+        s.value = res[1] + s.quad_x + s.offset;
+        s.i = 0;
+        s.reason = 0;
+        s.pg = HUGE_VAL;
+        s.pc = PC_PI;
+        break;
+      case PC_PI:
+        if (s.i >= s.niter) {
+          s.pc = PC_PF;
+          break;
+        }
+        s.k = 0;
+        s.pc = PC_PS;
+        break;
+      case PC_PS:
+        if (s.k < s.pcd_steps) {
+          issue(OP_PCD_STEP, 0, PC_PV);
+        } else {
+          s.op_r0 = NX;
+          s.op_r1 = MXV;
+          issue(OP_GRAD_PG, 0, PC_PE);
+        }
+        break;
+      case PC_PV:
+        s.k += 1;
+        s.op_r0 = NX;
+        s.op_r1 = MXV;
+        issue(OP_VIOL, F_CX, PC_PV2);
+        break;
+      case PC_PV2:
+        s.quad_x = .5 * res[0];
+        s.value = res[1] + s.quad_x + s.offset;
+        s.pc = PC_PS;
+        break;
+      case PC_PE:
+        s.pg = sqrt(res[0]);
+        s.diag[1] = sqrt(res[1]);
+        s.diag[2] = s.pg;
+        s.inner_total++;
+        s.i += 1;
+        if (s.value < s.max_value) {
+          s.reason = 1;
+          s.pc = PC_PF;
+        } else if (s.pg < s.max_pg) {
+          s.reason = 2;
+          s.pc = PC_PF;
+        } else {
+          s.pc = PC_PI;
+        }
+        break;
+      case PC_PF:
+        s.op_r0 = NX;
+        issue(OP_PERMUTE_OUT, 0, PC_PF2);
+        break;
+      case PC_PF2:
+        s.diag[0] = s.value;
+        s.diag[3] = 0;
+        s.diag[4] = (double)(s.reason ? s.i : s.i + 1);  // cd.c:369-378
+        s.ret = s.reason;
+        s.pc = PC_DONE;
+        break;
       // ------------------------------------------------------ approx_update
       case PC_U0:
         issue(OP_PERMUTE_PROBLEM, 0, PC_U1);

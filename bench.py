@@ -36,7 +36,9 @@ Legs after the timed loop run only while the wall-clock budget
 (--budget-s) allows; a leg that does not fit is reported as skipped.
 
 --workload setcover|mcf makes one ALM solve of the set-cover or
-multicommodity LP the step; --workload serve runs the request loop.
+multicommodity LP the step; --workload serve runs the request loop;
+--workload cd times parallel coordinate descent against the sequential
+device port of cd_solve (value: solves/s to a fixed objective).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload W]
 """
@@ -486,6 +488,110 @@ def run_single(args, lib):
     }
 
 
+def run_cd(args, lib):
+    """Parallel coordinate descent (acd_cd_solve_parallel, sample 1/16)
+    against the sequential device port of cd_solve on the plain FP64 LP
+    (m=2,000, n=5,000, 10 nonzeros per column): the first ALM subproblem
+    (lambda = 0, unit weights, the LP's cost) from x = 0.  The target is
+    the objective the sequential port reaches in 16 major steps (about 160
+    epochs of single-coordinate updates; srandom(1)).  A step is one
+    parallel solve to that target (max_value), seed = step number.  The
+    oracle's cd_solve (the reference's algorithm) runs to the same target
+    on one host core beside it."""
+    import pyoracle
+    from acdalm import instances
+    L = lib.lib
+    libc = C.CDLL(None)
+    lp = instances.config0(1)
+    mat = lib.make_matrix(lp, 0)
+    a = lib.approx_make(mat, lp)
+    x0 = np.zeros(lp.ncols)
+    seq = {}
+    for majors in (4, 16):
+        libc.srandom(1)
+        t = time.perf_counter()
+        _, _, d, _ = lib.cd_solve(a, x0, majors)
+        seq[majors] = (time.perf_counter() - t, d[0])
+        log("sequential device cd, %d major steps: %.3f s value %.10g"
+            % (majors, seq[majors][0], d[0]))
+    target = seq[16][1]
+    equal_updates = {}
+    for epochs in (40, 160):
+        _, _, d = lib.cd_solve_parallel(a, x0, epochs, seed=1)
+        equal_updates[epochs] = d[0]
+    names = [L.acd_pass_name(j).decode() for j in range(32)]
+
+    def one(seed):
+        t = time.perf_counter()
+        rc, _, d = lib.cd_solve_parallel(a, x0, 100000, seed=seed,
+                                         max_value=target)
+        return time.perf_counter() - t, rc, d
+
+    for w in range(args.warmup):
+        one(1000 + w)
+    lib.profile_bytes()
+    step_us, bytes_, launches, lat, epochs = np.zeros(32), np.zeros(32), 0, [], []
+    for k in range(args.steps):
+        dt, rc, d = one(k + 1)
+        if rc != 1:
+            raise RuntimeError("parallel cd missed the target: rc %d" % rc)
+        lat.append(dt)
+        epochs.append(int(d[4]))
+        for name, (cnt, sus, by) in lib.profile_bytes().items():
+            i = names.index(name)
+            step_us[i] += sus
+            bytes_[i] += by
+            launches += cnt
+        log("step %d: %.3f s, %d epochs, value %.10g" % (k, dt, d[4], d[0]))
+    total = float(np.sum(lat))
+    i = names.index("pcd_step")
+    achieved = bytes_[i] / (step_us[i] * 1e-6) / 1e9 if step_us[i] else 0.0
+    ora = pyoracle.Oracle()
+    om = ora.make_matrix(lp, 0)
+    oa = ora.approx_make(om, lp)
+    libc.srandom(1)
+    t = time.perf_counter()
+    _, _, d, _ = ora.cd_solve(oa, x0, 1000, max_value=target)
+    cpu_s = time.perf_counter() - t
+    ora.approx_free(oa)
+    ora.free_matrix(om)
+    lib.approx_free(a)
+    lib.free_matrix(mat)
+    lat_ms = np.array(lat) * 1e3
+    return {
+        "metric_note": "coordinate-descent solves/s to a fixed objective "
+                       "(the sequential port's after 16 major steps)",
+        "value": args.steps / total, "ms_per_step": total / args.steps * 1e3,
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "stop": "objective <= %.10g (max_value)" % target,
+        "target_objective": target, "epochs_to_target": epochs,
+        "sequential_device_port": {
+            "s_4_major": seq[4][0], "s_16_major": seq[16][0],
+            "value_4_major": seq[4][1], "value_16_major": seq[16][1],
+            "speedup_of_parallel": seq[16][0] / (total / args.steps)},
+        "parallel_at_equal_updates": {"40_epochs": equal_updates[40],
+                                      "160_epochs": equal_updates[160]},
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_step, pcd_step passes",
+            "launches": int(launches),
+            "share_of_step_kernel_time": round(step_us[i] / step_us.sum(), 3)
+            if step_us.sum() else None},
+        "cpu_baseline": {"value": 1.0 / cpu_s, "unit": "solves/s",
+                         "cores": 1, "kind": "port",
+                         "sample": "oracle cd_solve (the reference's "
+                                   "sequential coordinate descent), "
+                                   "srandom(1), to the same target, one "
+                                   "solve",
+                         "major_steps": int(d[4]), "seconds": cpu_s},
+        "workload": "cd_solve subproblem of the plain FP64 LP m=2000 "
+                    "n=5000 10 nnz/column (lambda=0, unit weights)",
+    }
+
+
 def serve_cpu_baseline(lps, large, arrivals, large_inner, cpu, sample=32,
                        large_iters=20):
     """The oracle replayed on the same arrival trace: a server with
@@ -640,7 +746,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["batch", "setcover", "mcf",
-                                           "serve"], default="batch")
+                                           "serve", "cd"], default="batch")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--stop-kkt", type=float, default=KKT_TOL,
                     help="stop once the relative KKT residual is at most "
@@ -702,12 +808,16 @@ def main():
     elif args.workload == "batch":
         r = run_batch(args, lib)
         steps, warmup = args.steps, args.warmup
+    elif args.workload == "cd":
+        r = run_cd(args, lib)
+        steps, warmup = args.steps, args.warmup
     else:
         r = run_single(args, lib)
         steps, warmup = args.steps, args.warmup
-    mixed = args.precision == "mixed"
+    mixed = args.precision == "mixed" and args.workload != "cd"
     line = {
-        "metric": METRIC, "value": r["value"], "unit": "solves/s",
+        "metric": r.get("metric_note", METRIC), "value": r["value"],
+        "unit": "solves/s",
         "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": r["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -718,16 +828,19 @@ def main():
                          "recurrence in FP64" if mixed else
                          "FP64 everywhere"),
         "data": "synthetic (seeded generator, acdalm.instances)",
-        "config": {"workload": r["workload"], "precision": args.precision,
-                   "stop": "relative KKT <= %g" % args.stop_kkt
-                           if args.stop_kkt > 0 else "reference rule",
+        "config": {"workload": r["workload"],
+                   "precision": "fp64" if args.workload == "cd"
+                                else args.precision,
+                   "stop": r.get("stop") or (
+                       "relative KKT <= %g" % args.stop_kkt
+                       if args.stop_kkt > 0 else "reference rule"),
                    "parallelism": "single GPU"},
         "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
         "roofline": r["roofline"], "cpu_baseline": r["cpu_baseline"],
         "wall_s": elapsed(),
     }
     line.update({k: v for k, v in r.items() if k not in line and
-                 k not in ("workload", "value")})
+                 k not in ("workload", "value", "metric_note", "stop")})
     return line
 
 

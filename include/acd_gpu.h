@@ -137,6 +137,26 @@ int alm_solve_ex(alm_t *, size_t niter, double * x, size_t nvars,
                  double * lambda, size_t nconstraints, FILE * log,
                  double * OUT_diagnosis, const struct acd_alm_options *);
 
+/* ---- parallel coordinate descent ---- */
+/* cd_solve's problem (minimise c'x + 1/2 sum_i w_i (A x - b)_i^2 + offset
+ * over l <= x <= u, from x) by parallel randomised coordinate descent on
+ * the device; no counterpart in the reference, whose cd_solve takes one
+ * coordinate at a time (and stays available, bit for bit).  Each step
+ * moves a random sample of the coordinates (each with probability
+ * `sample`, 0 < sample <= 1, drawn from `seed`) by the step of the
+ * expected separable overapproximation, x_j <- clamp(x_j - g_j / v'_j),
+ * v'_j = sum_i w_i a_ij^2 (1 + sample (omega_i - 1)), omega_i the nonzeros
+ * of row i.  niter counts epochs of
+ * ceil(1 / sample) steps.  Stops like cd_solve (value < max_value: 1,
+ * projected-gradient norm < max_pg: 2, else 0 after niter epochs);
+ * OUT_diagnosis (double[5], may be NULL) as cd_solve's: value, |g|,
+ * projected-gradient norm, 0, epochs.  Returns -1 for a sample outside
+ * (0, 1].  The result depends on the seed, not on the device. */
+int acd_cd_solve_parallel(double * x, size_t n, approx_t * approx,
+                          size_t niter, double max_pg, double max_value,
+                          double offset, double sample, uint64_t seed,
+                          double * OUT_diagnosis);
+
 /* ---- batched solves and the request loop ---- */
 /* A batched or served LP whose solve fails gets the negated code as its
  * return value (the single-LP entry points abort instead, as the

@@ -68,6 +68,10 @@ int ora_alm_free(ora_alm_t * a)
         return 0;
 }
 
+/* alm.c:23-43 */
+double * ora_alm_linear(ora_alm_t * a) { return a->linear; }
+ora_approx_t * ora_alm_approx(ora_alm_t * a) { return a->approx; }
+
 /* alm.c:98-104: plain sequential sum. */
 static double seq_dot(const double * x, const double * y, size_t n)
 {

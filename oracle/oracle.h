@@ -127,6 +127,10 @@ ora_alm_t * ora_alm_make(ora_matrix_t * constraints, size_t nrhs,
                          const double * upper, const double * lambda_lower,
                          const double * lambda_upper);
 int ora_alm_free(ora_alm_t *);
+/* alm.c:23-43 accessors: the LP cost (mutable) and the approx object that
+ * holds rhs, lower and upper (mutable through ora_approx_*). */
+double * ora_alm_linear(ora_alm_t *);
+ora_approx_t * ora_alm_approx(ora_alm_t *);
 int ora_alm_solve(ora_alm_t *, size_t niter, double * x, size_t nvars,
                   double * lambda, size_t nconstraints, FILE * log,
                   double * diag, const struct ora_alm_options *);

@@ -99,6 +99,8 @@ class Oracle:
                                         C.POINTER(AlmOptions)]),
             "ora_alm_default_options": (None, [C.POINTER(AlmOptions)]),
             "ora_alm_kkt": (C.c_double, [vp, dp, dp]),
+            "ora_alm_linear": (dp, [vp]),
+            "ora_alm_approx": (vp, [vp]),
             "ora_set_reverse_sums": (None, [C.c_int]),
         }
         for name, (res, args) in sig.items():

@@ -200,3 +200,82 @@ def test_alm_reaches_highs_optimum(oracle, name):
     assert abs(diag[0] - opt) <= 1e-6 * max(1.0, abs(opt)), (diag[0], opt)
     oracle.alm_free(alm)
     oracle.free_matrix(m)
+
+
+def test_kkt_closed_form(oracle):
+    """ora_alm_kkt on the hand-solved LP min x1 + 2 x2 s.t. x1 + x2 = 1,
+    0 <= x <= 10: zero at the optimum x = (1, 0), lambda = 1 (reduced costs
+    d = (0, 1)); at x = (0.9, 0) the primal residual is 0.1 / (1 + 1) =
+    0.05, the gap |0.9 - 1| / (1 + 0.9 + 1) = 0.0345, the dual residual 0,
+    so KKT = 0.05."""
+    lp = I.tiny_lp()
+    m = oracle.make_matrix(lp, 0)
+    a = oracle.alm_make(m, lp)
+    assert oracle.alm_kkt(a, np.array([1.0, 0.0]), np.array([1.0])) == 0.0
+    k = oracle.alm_kkt(a, np.array([0.9, 0.0]), np.array([1.0]))
+    assert abs(k - 0.05) <= 1e-15, k
+    # lambda = 1.5: d = (-0.5, 0.5); x = (1, 0) is feasible (r_p = 0);
+    # clamp(x - d) = (1.5, 0), so r_d = 0.5 / (1 + sqrt(5)); the dual bound
+    # b lambda + sum(l d+ - u d-) = 1.5 - 10 * 0.5 = -3.5 against c'x = 1
+    # gives r_gap = 4.5 / (1 + 1 + 3.5) = 9/11, the maximum.
+    k = oracle.alm_kkt(a, np.array([1.0, 0.0]), np.array([1.5]))
+    assert abs(k - 9.0 / 11.0) <= 1e-15, k
+    oracle.alm_free(a)
+    oracle.free_matrix(m)
+
+
+def test_warm_start_equals_the_reference_build(oracle):
+    """Edits through the accessors, approx_update and a warm approx_solve,
+    then an ALM warm re-solve after editing costs and bounds: the oracle
+    and the reference built from its own sources agree bit for bit."""
+    import dataclasses
+    import pyoracle
+    from acdalm.capi import CLib
+    if not os.path.exists(pyoracle.REF_APPROX_SO):
+        pytest.skip("reference build absent (oracle/_ref)")
+    ref = CLib(pyoracle.REF_APPROX_SO, gpu_extras=False)
+    lp = SOLVE["small"]
+    rng = np.random.default_rng(7)
+    w = np.linspace(0.5, 2.0, lp.nrows)
+    rhs2 = lp.rhs * 1.05
+    lin2 = lp.linear + rng.uniform(-0.3, 0.3, lp.ncols)
+    up2 = lp.upper.copy()
+    up2[rng.choice(lp.ncols, 10, replace=False)] = 7.5
+    view = lambda p, n: np.ctypeslib.as_array(p, (n,))
+    mr, mo = ref.make_matrix(lp, 0), oracle.make_matrix(lp, 0)
+    ar, ao = ref.approx_make(mr, lp, weight=w), oracle.approx_make(mo, lp, weight=w)
+    r1 = ref.approx_solve(ar, np.zeros(lp.ncols), 60)
+    o1 = oracle.approx_solve(ao, np.zeros(lp.ncols), 60)
+    assert r1[0] == o1[0] and bit_equal(r1[1], o1[1])
+    R, O = ref.lib, oracle.lib
+    for gr, go, vals, n in [(R.approx_rhs, O.ora_approx_rhs, rhs2, lp.nrows),
+                            (R.approx_linear, O.ora_approx_linear, lin2, lp.ncols),
+                            (R.approx_upper, O.ora_approx_upper, up2, lp.ncols)]:
+        view(gr(ar), n)[:] = vals
+        view(go(ao), n)[:] = vals
+    R.approx_update(ar)
+    O.ora_approx_update(ao)
+    r2 = ref.approx_solve(ar, r1[1], 60)
+    o2 = oracle.approx_solve(ao, o1[1], 60)
+    assert r2[0] == o2[0] and bit_equal(r2[1], o2[1])
+    assert bit_equal(r2[2], o2[2])
+    ref.approx_free(ar)
+    oracle.approx_free(ao)
+    # ALM warm re-solve
+    lr, lo = ref.alm_make(mr, lp), oracle.alm_make(mo, lp)
+    c1 = ref.alm_solve(lr, 1000, np.zeros(lp.ncols), np.zeros(lp.nrows))
+    d1 = oracle.alm_solve(lo, 1000, np.zeros(lp.ncols), np.zeros(lp.nrows))
+    assert bit_equal(c1[1], d1[1]) and bit_equal(c1[2], d1[2])
+    view(R.alm_linear(lr), lp.ncols)[:] = lin2
+    view(R.alm_upper(lr), lp.ncols)[:] = up2
+    view(O.ora_alm_linear(lo), lp.ncols)[:] = lin2
+    view(O.ora_approx_upper(O.ora_alm_approx(lo)), lp.ncols)[:] = up2
+    c2 = ref.alm_solve(lr, 1000, c1[1], c1[2])
+    d2 = oracle.alm_solve(lo, 1000, d1[1], d1[2])
+    assert c2[0] == d2[0]
+    assert bit_equal(c2[1], d2[1]) and bit_equal(c2[2], d2[2])
+    assert bit_equal(c2[3], d2[3])
+    ref.alm_free(lr)
+    oracle.alm_free(lo)
+    ref.free_matrix(mr)
+    oracle.free_matrix(mo)

@@ -210,12 +210,16 @@ constexpr bool kBatchCtlCall = true;
 constexpr int kLdsPairs = 4;
 __device__ constexpr int kLdsPair[kLdsPairs][3] = {
     {-1, NZ, NZP}, {1, MYV, MSY}, {1, MZV, MZPV}, {1, MXV, MSX}};
-// Per-workgroup LDS for resident vectors: 3 LP workgroups per CU (the
-// kernel's register-bound occupancy) fit in the 160 KB of LDS with their
-// static records (~1.5 KB each).
-// Measured on 4,096 LPs: 51 KB (3 per CU) 387 solves/s, 76 KB (2 per CU)
-// 365, 150 KB (1 per CU) 281; without residency 345.
-constexpr uint32_t kLdsBudget = 51u << 10;
+// Per-workgroup LDS, static records plus resident vectors: 3 LP
+// workgroups per CU (the kernel's register-bound occupancy) fit in the
+// 160 KB of LDS when each takes at most 53,760 bytes (52.5 KB, a whole
+// number of allocation granules); the resident vectors get what the
+// static records (~1.5 KB) leave.  One static record 16 bytes larger
+// (53,776 bytes in all) dropped the kernel to 2 workgroups per CU: 350
+// against 388 solves/s on 4,096 LPs.  Measured with a 51 KB vector
+// budget: 3 per CU 387 solves/s, 76 KB (2 per CU) 365, 150 KB (1 per CU)
+// 281; without residency 345.
+constexpr uint32_t kLdsPerWorkgroup = 53760;
 
 extern __shared__ __align__(16) unsigned char lds_vecs[];
 
@@ -251,14 +255,15 @@ __device__ void lds_copy(const Args& la, const State& ls, uint32_t resident,
 // LDS bytes and pair mask for an LP of m rows and n columns with
 // si-byte iterates under the budget: pairs are taken in order, each one
 // that still fits.
-uint32_t lds_plan(uint32_t m, uint32_t n, uint32_t si, uint32_t* bytes) {
+uint32_t lds_plan(uint32_t m, uint32_t n, uint32_t si, uint32_t budget,
+                  uint32_t* bytes) {
   const size_t bn = 2 * (((size_t)n * si + 15) & ~(size_t)15);
   const size_t bm = 2 * (((size_t)m * si + 15) & ~(size_t)15);
   const size_t need[kLdsPairs] = {bn, bm, bm, bm};
   size_t used = 0;
   uint32_t mask = 0;
   for (int q = 0; q < kLdsPairs; q++) {
-    if (used + need[q] > kLdsBudget) continue;
+    if (used + need[q] > budget) continue;
     used += need[q];
     mask |= 1u << q;
   }
@@ -340,7 +345,7 @@ k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
         g.gwave = v;
         g.nwaves = kVWaves;
         Ops<BlockG, VI, VC, MT> ops(g, la, ls);
-        ops.run();
+        ops.template run<true>();
 #pragma unroll
         for (int k = 0; k < kRes; k++) {
           const double r = wave_reduce(ops.p[k], (mm >> k) & 1);
@@ -362,7 +367,7 @@ k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
       t_ops += t1 - t_mark;
       if (ls.op != OP_NONE) moved += (uint64_t)alg_bytes(ls.op, ls.op_flags, la);
       Ctl c{ls, la, res};
-      c.advance<kBatchCtlCall>();
+      c.advance<kBatchCtlCall, true>();
       t_mark = wall_clock64();
       t_ctl += t_mark - t1;
     }
@@ -413,6 +418,27 @@ k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
 }
 
 __global__ void k_stamp(uint64_t* t) { *t = wall_clock64(); }
+
+// LDS left for resident vectors: the per-workgroup total less the batched
+// kernel's static LDS (its records do not depend on the instantiation;
+// the largest of two is taken), in 16-byte units; 0 turns residency off.
+uint32_t lds_vector_budget() {
+  static const uint32_t budget = [] {
+    const void* fns[2] = {
+        reinterpret_cast<const void*>(&k_batch<double, double, double, 4>),
+        reinterpret_cast<const void*>(&k_batch<double, float, uint16_t, 1>)};
+    size_t st = 0;
+    for (const void* f : fns) {
+      hipFuncAttributes fa;
+      if (hipFuncGetAttributes(&fa, f) != hipSuccess) return 0u;
+      if (fa.sharedSizeBytes > st) st = fa.sharedSizeBytes;
+    }
+    return st + 1024 < kLdsPerWorkgroup
+               ? (uint32_t)((kLdsPerWorkgroup - st) & ~(size_t)15)
+               : 0u;
+  }();
+  return budget;
+}
 
 Sell view(const acd_sell& op) {
   Sell v;
@@ -1234,7 +1260,7 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
       h_args[i].partials = nullptr;
       if (lds_on && q->mode == ACD_MODE_ALM)
         h_args[i].resident = lds_plan(lps[i]->m, lps[i]->n, h_args[i].si,
-                                      &lds_bytes[i]);
+                                      lds_vector_budget(), &lds_bytes[i]);
       acd_solve_request qi = *q;
       if (!logged) qi.log = 0;
       init_state(&h_st[i], lps[i], &qi);

@@ -420,6 +420,7 @@ struct Ops {
     return minpd((double)P(a.p_u)[j], t);
   }
 
+  template <bool SolveOnly = false>
   __device__ __forceinline__ void run() {
     const uint32_t fl = s.op_flags;
     const VC* pc = P(a.p_c);
@@ -691,47 +692,6 @@ struct Ops {
           each(a.n, [&](uint32_t j) { p[0] += (double)pc[j] * (double)C[j]; });
         break;
       }
-      case OP_MPG_GRAD0: {  // approx/approx.c:538-539
-        VI* G1 = VN(NG);
-        VI* G2 = VN(NG2);
-        spmv_rows<1>(a.AT, VM(MXV), nullptr, [&](uint32_t j, double gx, double) {
-          const VI gg = (VI)(gx + (double)pc[j]);
-          G1[j] = gg;
-          G2[j] = gg;
-        });
-        break;
-      }
-      case OP_MPG_GRAD: {  // approx/mpg.inc:161-174
-        const VI* X = VN(NX);
-        const VI* Z = VN(NZ);
-        VI* G1 = VN(NG);
-        const VI* G2 = VN(NG2);
-        spmv_rows<1>(a.AT, VM(MXV), nullptr, [&](uint32_t j, double gx, double) {
-          const VI ggv = (VI)(gx + (double)pc[j]);
-          const double gg = (double)ggv;
-          G1[j] = ggv;
-          const double xj = (double)X[j];
-          const double d = xj - clamp_lu(j, xj - gg);
-          p[0] += d * d;
-          const double yj = xj - (double)Z[j];
-          const double uj = gg - (double)G2[j];
-          p[1] += yj * yj;
-          p[2] += yj * uj;
-        });
-        break;
-      }
-      case OP_MPG_ZUPD: {  // approx/mpg.inc:180-182
-        const VI* X = VN(NX);
-        VI* Z = VN(NZ);
-        const VI* G1 = VN(NG);
-        const double st = s.op_a;
-        each(a.n, [&](uint32_t j) {
-          const double t = (double)X[j] - st * (double)G1[j];
-          const double lo = maxpd((double)P(a.p_l)[j], t);
-          Z[j] = (VI)minpd(lo, (double)P(a.p_u)[j]);
-        });
-        break;
-      }
       case OP_ALM_LAMPROJ: {  // alm.c:171-180, :314
         each(a.m, [&](uint32_t i) {
           a.lam[i] = fmax(a.lam_lo[i], fmin(a.lam[i], a.lam_hi[i]));
@@ -843,6 +803,61 @@ struct Ops {
         each(a.m, [&](uint32_t i) {
           p[4] += a.rhs[i] * a.lam[i];
           p[5] += a.rhs[i] * a.rhs[i];
+        });
+        break;
+      }
+      default:
+        if constexpr (!SolveOnly) run_rest();
+        break;
+    }
+  }
+
+  // The ops of the SPG and parallel coordinate descent solvers, which the
+  // batched kernel (ALM with the APPROX inner solver only) leaves out, so
+  // that its code and the control continuation it calls (57 KB together)
+  // stay within the 64 KB instruction cache that two compute units share.
+  __device__ __forceinline__ void run_rest() {
+    const uint32_t fl = s.op_flags;
+    const VC* pc = P(a.p_c);
+    switch (s.op) {
+      case OP_MPG_GRAD0: {  // approx/approx.c:538-539
+        VI* G1 = VN(NG);
+        VI* G2 = VN(NG2);
+        spmv_rows<1>(a.AT, VM(MXV), nullptr, [&](uint32_t j, double gx, double) {
+          const VI gg = (VI)(gx + (double)pc[j]);
+          G1[j] = gg;
+          G2[j] = gg;
+        });
+        break;
+      }
+      case OP_MPG_GRAD: {  // approx/mpg.inc:161-174
+        const VI* X = VN(NX);
+        const VI* Z = VN(NZ);
+        VI* G1 = VN(NG);
+        const VI* G2 = VN(NG2);
+        spmv_rows<1>(a.AT, VM(MXV), nullptr, [&](uint32_t j, double gx, double) {
+          const VI ggv = (VI)(gx + (double)pc[j]);
+          const double gg = (double)ggv;
+          G1[j] = ggv;
+          const double xj = (double)X[j];
+          const double d = xj - clamp_lu(j, xj - gg);
+          p[0] += d * d;
+          const double yj = xj - (double)Z[j];
+          const double uj = gg - (double)G2[j];
+          p[1] += yj * yj;
+          p[2] += yj * uj;
+        });
+        break;
+      }
+      case OP_MPG_ZUPD: {  // approx/mpg.inc:180-182
+        const VI* X = VN(NX);
+        VI* Z = VN(NZ);
+        const VI* G1 = VN(NG);
+        const double st = s.op_a;
+        each(a.n, [&](uint32_t j) {
+          const double t = (double)X[j] - st * (double)G1[j];
+          const double lo = maxpd((double)P(a.p_l)[j], t);
+          Z[j] = (VI)minpd(lo, (double)P(a.p_u)[j]);
         });
         break;
       }
@@ -1028,20 +1043,26 @@ struct Ctl {
   // continuation's temporaries beside its own (the batched kernel, whose
   // occupancy the register count sets, uses it; the control kernel
   // inlines it).
-  template <bool OutOfLine = false>
+  // SolveOnly = true keeps only the APPROX, ALM and approx_update
+  // continuations (any other program counter fails the solve).
+  template <bool OutOfLine = false, bool SolveOnly = false>
   __device__ void advance() {
     s.op = OP_NONE;
     for (int guard = 0; guard < 256 && s.op == OP_NONE && s.pc != PC_DONE;
          guard++) {
       if (OutOfLine)
-        step_call();
+        step_call<SolveOnly>();
       else
-        step();
+        step<SolveOnly>();
     }
   }
 
-  __device__ __attribute__((noinline)) void step_call() { step(); }
+  template <bool SolveOnly>
+  __device__ __attribute__((noinline)) void step_call() {
+    step<SolveOnly>();
+  }
 
+  template <bool SolveOnly = false>
   __device__ void step() {
     switch (s.pc) {
       // ------------------------------------------------------------ APPROX
@@ -1311,86 +1332,6 @@ struct Ctl {
         s.ret = s.reason;
         s.pc = (s.mode == ACD_MODE_ALM) ? PC_LE0 : PC_DONE;
         break;
-      // ------------------------------------------------------------ SPG
-      case PC_M0:  // approx/approx.c:534-541
-        issue(OP_INIT_X, 0, PC_M1);
-        break;
-      case PC_M1:
-        s.op_r0 = NX;
-        s.op_r1 = MXV;
-        issue(OP_VIOL, 0, PC_M2);
-        break;
-      case PC_M2:
-        s.quad_x = .5 * res[0];
-        issue(OP_MPG_GRAD0, 0, PC_M3);
-        break;
-      case PC_M3:
-        s.i = 0;
-        s.reason = 0;
-        s.pc = PC_MI;
-        break;
-      case PC_MI:  // approx/approx.c:543-545; mpg.inc:155-199
-        if (s.i >= s.niter || s.reason) {
-          s.op_r0 = NX;
-          issue(OP_PERMUTE_OUT, F_CX, PC_MF1);
-          break;
-        }
-        if (a.ev_cap && s.events + 8 > a.ev_cap) {
-          s.events_full = 1;
-          s.op = OP_NONE;
-          return;
-        }
-        issue(OP_MPG_GRAD, 0, PC_MI2);
-        break;
-      case PC_MI2: {
-        s.pg = sqrt(res[0]);
-        double st = res[1] / res[2];
-        if (isnan(st)) st = 1;
-        if (!(st > 1e8)) {
-          issue(OP_MPG_ZUPD, 0, PC_MI3, 0, st);
-        } else {
-          s.op_r0 = NX;
-          s.op_r1 = MXV;
-          issue(OP_VIOL, F_CX, PC_MI5);
-        }
-        break;
-      }
-      case PC_MI3:
-        swap_(s.pn[NZ], s.pn[NX]);
-        swap_(s.pn[NG2], s.pn[NG]);
-        s.op_r0 = NX;
-        s.op_r1 = MXV;
-        issue(OP_VIOL, F_CX, PC_MI5);
-        break;
-      case PC_MI5: {  // approx/approx.c:546-553
-        s.quad_x = .5 * res[0];
-        const double value = s.offset + (res[1] + s.quad_x);
-        if (s.pg < s.max_pg) s.reason = 2;
-        if (value < s.max_value) s.reason = 1;
-        if (s.log &&
-            ((s.period && (s.i % s.period == 0)) || (s.i == 0) || s.reason))
-          emit(ACD_EV_MPG, s.i + 1, s.pg, value);
-        s.value = value;
-        s.inner_total++;
-        s.i += 1;
-        s.pc = PC_MI;
-        break;
-      }
-      case PC_MF1:  // approx/approx.c:559-569
-        s.cx = res[0];
-        s.op_r0 = NX;
-        s.op_r1 = MXV;
-        issue(OP_GRAD_PG, 0, PC_MF2);
-        break;
-      case PC_MF2:
-        s.diag[0] = s.offset + (s.cx + s.quad_x);
-        s.diag[1] = sqrt(res[1]);
-        s.diag[2] = sqrt(res[0]);
-        s.diag[3] = HUGE_VAL;
-        s.diag[4] = (double)(s.i + 1);
-        s.ret = 2;
-        s.pc = PC_DONE;
-        break;
       // ------------------------------------------------------------ ALM
       case PC_L0:  // alm.c:289-300
         issue(OP_ALM_LAMPROJ, 0, PC_L1);
@@ -1514,6 +1455,109 @@ struct Ctl {
         }
         break;
       }
+      // ------------------------------------------------------ approx_update
+      case PC_U0:
+        issue(OP_PERMUTE_PROBLEM, 0, PC_U1);
+        break;
+      case PC_U1:
+        issue(OP_STEP_SIZES, 0, PC_U2);
+        break;
+      case PC_U2:
+        s.pc = PC_DONE;
+        break;
+      default:
+        if constexpr (SolveOnly)
+          fail(ACD_ERR_LAUNCH);
+        else
+          step_rest();
+        break;
+    }
+  }
+
+  // The SPG and parallel coordinate descent continuations (left out of
+  // the batched kernel with their ops).
+  __device__ void step_rest() {
+    switch (s.pc) {
+      // ------------------------------------------------------------ SPG
+      case PC_M0:  // approx/approx.c:534-541
+        issue(OP_INIT_X, 0, PC_M1);
+        break;
+      case PC_M1:
+        s.op_r0 = NX;
+        s.op_r1 = MXV;
+        issue(OP_VIOL, 0, PC_M2);
+        break;
+      case PC_M2:
+        s.quad_x = .5 * res[0];
+        issue(OP_MPG_GRAD0, 0, PC_M3);
+        break;
+      case PC_M3:
+        s.i = 0;
+        s.reason = 0;
+        s.pc = PC_MI;
+        break;
+      case PC_MI:  // approx/approx.c:543-545; mpg.inc:155-199
+        if (s.i >= s.niter || s.reason) {
+          s.op_r0 = NX;
+          issue(OP_PERMUTE_OUT, F_CX, PC_MF1);
+          break;
+        }
+        if (a.ev_cap && s.events + 8 > a.ev_cap) {
+          s.events_full = 1;
+          s.op = OP_NONE;
+          return;
+        }
+        issue(OP_MPG_GRAD, 0, PC_MI2);
+        break;
+      case PC_MI2: {
+        s.pg = sqrt(res[0]);
+        double st = res[1] / res[2];
+        if (isnan(st)) st = 1;
+        if (!(st > 1e8)) {
+          issue(OP_MPG_ZUPD, 0, PC_MI3, 0, st);
+        } else {
+          s.op_r0 = NX;
+          s.op_r1 = MXV;
+          issue(OP_VIOL, F_CX, PC_MI5);
+        }
+        break;
+      }
+      case PC_MI3:
+        swap_(s.pn[NZ], s.pn[NX]);
+        swap_(s.pn[NG2], s.pn[NG]);
+        s.op_r0 = NX;
+        s.op_r1 = MXV;
+        issue(OP_VIOL, F_CX, PC_MI5);
+        break;
+      case PC_MI5: {  // approx/approx.c:546-553
+        s.quad_x = .5 * res[0];
+        const double value = s.offset + (res[1] + s.quad_x);
+        if (s.pg < s.max_pg) s.reason = 2;
+        if (value < s.max_value) s.reason = 1;
+        if (s.log &&
+            ((s.period && (s.i % s.period == 0)) || (s.i == 0) || s.reason))
+          emit(ACD_EV_MPG, s.i + 1, s.pg, value);
+        s.value = value;
+        s.inner_total++;
+        s.i += 1;
+        s.pc = PC_MI;
+        break;
+      }
+      case PC_MF1:  // approx/approx.c:559-569
+        s.cx = res[0];
+        s.op_r0 = NX;
+        s.op_r1 = MXV;
+        issue(OP_GRAD_PG, 0, PC_MF2);
+        break;
+      case PC_MF2:
+        s.diag[0] = s.offset + (s.cx + s.quad_x);
+        s.diag[1] = sqrt(res[1]);
+        s.diag[2] = sqrt(res[0]);
+        s.diag[3] = HUGE_VAL;
+        s.diag[4] = (double)(s.i + 1);
+        s.ret = 2;
+        s.pc = PC_DONE;
+        break;
       // ------------------------------------------- parallel coordinate descent
       // Epochs of pcd_steps sampled steps each; after every step the
       // violation and value at x are recomputed; after every epoch the
@@ -1591,16 +1635,6 @@ struct Ctl {
         s.diag[3] = 0;
         s.diag[4] = (double)(s.reason ? s.i : s.i + 1);  // cd.c:369-378
         s.ret = s.reason;
-        s.pc = PC_DONE;
-        break;
-      // ------------------------------------------------------ approx_update
-      case PC_U0:
-        issue(OP_PERMUTE_PROBLEM, 0, PC_U1);
-        break;
-      case PC_U1:
-        issue(OP_STEP_SIZES, 0, PC_U2);
-        break;
-      case PC_U2:
         s.pc = PC_DONE;
         break;
       default:

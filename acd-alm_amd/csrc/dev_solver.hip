@@ -25,7 +25,7 @@ inline hipStream_t S() { return (hipStream_t)acd_dev_stream(); }
 
 constexpr int kWPB = 4;  // wavefronts per workgroup of the step kernel
 constexpr uint32_t kEventCap = 4096;
-constexpr int kPairs = 32;  // (step, control) pairs per graph
+constexpr int kPairs = 32;  // ops per graph (fused launches or step/control pairs)
 constexpr uint32_t kMaxBlocks = 2048;  // step-kernel grid cap (control loads)
 // Precision keys: iterate vectors / constant vectors / matrix values.
 enum {
@@ -84,45 +84,80 @@ __global__ void __launch_bounds__(64 * kWPB) k_step(Args a) {
 // block b at k * nblocks + b; wave w owns slots 2w and 2w+1; lane l sums
 // blocks l, l+64, ... in that order, then the lanes combine by butterfly),
 // then advances the state machine on one thread with the state record held
-// in LDS.  Every load (state words and all partials) is issued before the
-// first wait, so the kernel pays one memory latency instead of one per
-// batch.
-__global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
-  __shared__ double res[kRes];
-  __shared__ State ls;
-  const uint64_t t_entry = wall_clock64();
+// in LDS.  Called by a workgroup of 256 threads: by the control kernel, and
+// by the last workgroup of the fused step kernel.  The partials are loaded
+// kChunk per lane and slot at a time; the combining order does not depend
+// on kChunk.  With kChunk = kMaxBlocks / 64 every load (state words and all
+// partials) is issued before the first wait, so the control kernel pays one
+// memory latency instead of one per batch; the fused kernel takes smaller
+// chunks so that its register count stays that of the step.
+template <int kChunk, bool OutOfLine>
+__device__ __forceinline__ void control_block(const Args& a, uint32_t nblocks,
+                                              State& ls, double* res,
+                                              uint64_t t_entry) {
   State* st = a.st;
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int kPer = kMaxBlocks / 64;
-  double v0[kPer], v1[kPer];
+  static_assert(kPer % kChunk == 0, "chunks tile the partials");
   // The per-op counters (thread t holds word t of the four arrays) and
   // the phase timers (thread 0) are read in the same round as the
-  // partials, so updating them costs no extra memory round trip.
+  // state, so updating them costs no extra memory round trip.
   uint64_t* pw = reinterpret_cast<uint64_t*>(a.prof);
   uint64_t pf = 0, cp[4] = {0, 0, 0, 0};
-  {
-    if (threadIdx.x < 128) pf = pw[threadIdx.x];
-    if (threadIdx.x == 0)
-      for (int k = 0; k < 4; k++) cp[k] = a.prof->ctl_phase[k];
-    const double* p0 = a.partials + (size_t)(2 * w) * nblocks;
-    const double* p1 = p0 + nblocks;
+  if (threadIdx.x < 128) pf = pw[threadIdx.x];
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; k++) cp[k] = a.prof->ctl_phase[k];
+  const double* p0 = a.partials + (size_t)(2 * w) * nblocks;
+  const double* p1 = p0 + nblocks;
+  double v0[kChunk], v1[kChunk];
+  auto load = [&](int j0) {
 #pragma unroll
-    for (int j = 0; j < kPer; j++) {
-      const uint32_t b = lane + 64 * j;
+    for (int j = 0; j < kChunk; j++) {
+      const uint32_t b = lane + 64 * (j0 + j);
       v0[j] = b < nblocks ? p0[b] : 0.0;
       v1[j] = b < nblocks ? p1[b] : 0.0;
     }
+  };
+  load(0);  // issued together with the state loads
+  {
     const uint64_t* src = reinterpret_cast<const uint64_t*>(st);
     uint64_t* dst = reinterpret_cast<uint64_t*>(&ls);
     for (uint32_t i = threadIdx.x; i < sizeof(State) / 8; i += 256)
       dst[i] = src[i];
   }
   __syncthreads();
-  const uint64_t t_loaded = wall_clock64();
   const uint32_t mm = ls.op_maxmask;
   const int32_t op = ls.op;
   const bool ran = op != OP_NONE;
   const bool live = ls.pc != PC_DONE && !ls.error;
+  const bool mx0 = (mm >> (2 * w)) & 1, mx1 = (mm >> (2 * w + 1)) & 1;
+  double acc0 = mx0 ? -HUGE_VAL : 0.0, acc1 = mx1 ? -HUGE_VAL : 0.0;
+  if (ran && lane < nblocks) {
+#pragma unroll 1
+    for (int j0 = 0; j0 < kPer; j0 += kChunk) {
+      if (j0 > 0) {
+        if (lane + 64u * j0 >= nblocks) break;
+        load(j0);
+      }
+#pragma unroll
+      for (int j = 0; j < kChunk; j++) {
+        if (j0 + j == 0) {
+          acc0 = v0[0];
+          acc1 = v1[0];
+        } else if (lane + 64u * (j0 + j) < nblocks) {
+          acc0 = combine(acc0, v0[j], mx0);
+          acc1 = combine(acc1, v1[j], mx1);
+        }
+      }
+    }
+  }
+  const uint64_t t_loaded = wall_clock64();
+  acc0 = wave_reduce(acc0, mx0);
+  acc1 = wave_reduce(acc1, mx1);
+  if (lane == 0) {
+    res[2 * w] = acc0;
+    res[2 * w + 1] = acc1;
+  }
   // Per-op counters: the four threads holding this op's words.
   if (threadIdx.x < 128 && (threadIdx.x & 31) == (uint32_t)op && live &&
       ran && op >= 0 && op < 32) {
@@ -135,26 +170,11 @@ __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
     }
     pw[threadIdx.x] = pf + d;
   }
-#pragma unroll
-  for (int kk = 0; kk < 2; kk++) {
-    const int k = 2 * w + kk;
-    const bool mx = (mm >> k) & 1;
-    const double* v = kk ? v1 : v0;
-    double acc = mx ? -HUGE_VAL : 0.0;
-    if (ran && lane < nblocks) {
-      acc = v[0];
-#pragma unroll
-      for (int j = 1; j < kPer; j++)
-        if (lane + 64 * j < nblocks) acc = combine(acc, v[j], mx);
-    }
-    acc = wave_reduce(acc, mx);
-    if (lane == 0) res[k] = acc;
-  }
   __syncthreads();
   if (threadIdx.x == 0 && live) {
     const uint64_t t_comb = wall_clock64();
     Ctl c{ls, a, res};
-    c.advance();
+    c.advance<OutOfLine>();
     ls.t_last = wall_clock64();
     if (ran) {
       a.prof->ctl_phase[0] = cp[0] + (t_loaded - t_entry);
@@ -172,6 +192,76 @@ __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
   }
 }
 
+__global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
+  __shared__ double res[kRes];
+  __shared__ State ls;
+  const uint64_t t_entry = wall_clock64();
+  control_block<kMaxBlocks / 64, false>(a, nblocks, ls, res, t_entry);
+}
+
+// The step and the control step in one launch: every workgroup runs its
+// share of the op and writes its partials, then takes a ticket; the last
+// workgroup to arrive (its ticket equals the grid size less one) sees
+// every partial (release/acquire at device scope around the ticket),
+// resets the ticket and runs the control step exactly as the control
+// kernel does, so the results are the same bit for bit and one launch and
+// one kernel boundary per op disappear.  Only wave 0 wrote what the last
+// workgroup reads (the partials, and block 0's start time), so only it
+// releases.  Measured on the set-cover LP it is still slower than the
+// (step, control) pair: a device-scope release writes back the whole L2
+// of the XCD, once per workgroup (off by default; ACD_SOLVER_FUSED=1).
+// Relaxed device-scope atomics alone (no fences) do not make the partials
+// visible across XCDs here: the control step read stale values.  An op of
+// OP_NONE runs nothing here (the host issues a control kernel if a live
+// solve is ever left without an op).
+template <class VI, class VC, class MT>
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(4)))
+k_fused(Args a, uint32_t* ticket) {
+  static_assert(64 * kWPB == 256, "the control step runs on 256 threads");
+  const State& s = *a.st;
+  if (s.op == OP_NONE) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.st->t_step = wall_clock64();
+  __shared__ double lds[kWPB][kRes];
+  __shared__ double res[kRes];
+  __shared__ State ls;
+  __shared__ uint32_t last;
+  {
+    GridG g;
+    g.lane = threadIdx.x & 63;
+    g.gwave = blockIdx.x * kWPB + (threadIdx.x >> 6);
+    g.nwaves = gridDim.x * kWPB;
+    Ops<GridG, VI, VC, MT> ops(g, a, s);
+    ops.run();
+    const uint32_t mm = s.op_maxmask;
+#pragma unroll
+    for (int k = 0; k < kRes; k++) {
+      const double v = wave_reduce(ops.p[k], (mm >> k) & 1);
+      if (g.lane == 0) lds[threadIdx.x >> 6][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kRes) {
+      const int k = threadIdx.x;
+      const bool mx = (mm >> k) & 1;
+      double acc = lds[0][k];
+      for (int w = 1; w < kWPB; w++) acc = combine(acc, lds[w][k], mx);
+      a.partials[(size_t)k * gridDim.x + blockIdx.x] = acc;
+    }
+  }
+  if (threadIdx.x < 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (threadIdx.x == 0)
+      last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1
+                 ? 1u
+                 : 0u;
+  }
+  __syncthreads();
+  if (!last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (threadIdx.x == 0) *ticket = 0u;
+  control_block<8, true>(a, gridDim.x, ls, res, wall_clock64());
+}
+
 // ---------------------------------------------------------------------------
 // Batched solves: one workgroup per LP runs the same ops and control
 // continuation with the workgroup as the group and __syncthreads as the
@@ -186,7 +276,7 @@ __global__ void __launch_bounds__(256) k_ctl(Args a, uint32_t nblocks) {
 // waves (1, 2 or 4) runs virtual waves w, w + WPB, ... one after the
 // other, so the sums, and the LP's whole trajectory, are the same for
 // every WPB: the workgroup size is a pure speed choice.
-constexpr uint32_t kBatchOps = 1u << 16;
+constexpr uint32_t kBatchOps = 1u << 22;
 constexpr uint32_t kVWaves = 4;
 
 struct BlockG {
@@ -224,23 +314,45 @@ constexpr uint32_t kLdsPerWorkgroup = 53760;
 extern __shared__ __align__(16) unsigned char lds_vecs[];
 
 // Copies the resident slots between their global homes and LDS
-// (to_lds) or back; slot order as assigned by k_batch.
+// (to_lds) or back; slot order as assigned by k_batch.  Each slot is
+// copied in 16-byte words, four per thread in flight: a slot's LDS size
+// is its length rounded up to 16 bytes, which the global vector's spare
+// element covers (vectors hold length + 1 doubles), and the padding word
+// goes back unchanged (ops write only indices below the length).
 template <int WPB>
 __device__ void lds_copy(const Args& la, const State& ls, uint32_t resident,
                          void* const* home, bool to_lds) {
   size_t off = 0;
   uint32_t k = 0;
+  constexpr uint32_t kT = 64 * WPB;
   for (int q = 0; q < kLdsPairs; q++) {
     if (!((resident >> q) & 1)) continue;
     const bool nside = kLdsPair[q][0] < 0;
     const uint32_t len = nside ? la.n : la.m;
     const size_t bytes = (len * (size_t)la.si + 15) & ~(size_t)15;
+    const uint32_t words = (uint32_t)(bytes / 16);
     for (int e = 0; e < 2; e++) {
-      // 4-byte words (iterates are 8 or 4 bytes)
-      uint32_t* g = reinterpret_cast<uint32_t*>(home[k]);
-      uint32_t* l = reinterpret_cast<uint32_t*>(lds_vecs + off);
-      const uint32_t words = (uint32_t)(len * (size_t)la.si / 4);
-      for (uint32_t i = threadIdx.x; i < words; i += 64 * WPB) {
+      uint4* g = reinterpret_cast<uint4*>(home[k]);
+      uint4* l = reinterpret_cast<uint4*>(lds_vecs + off);
+      uint32_t i = threadIdx.x;
+      for (; i + 3 * kT < words; i += 4 * kT) {
+        if (to_lds) {
+          const uint4 a0 = g[i], a1 = g[i + kT], a2 = g[i + 2 * kT],
+                      a3 = g[i + 3 * kT];
+          l[i] = a0;
+          l[i + kT] = a1;
+          l[i + 2 * kT] = a2;
+          l[i + 3 * kT] = a3;
+        } else {
+          const uint4 a0 = l[i], a1 = l[i + kT], a2 = l[i + 2 * kT],
+                      a3 = l[i + 3 * kT];
+          g[i] = a0;
+          g[i + kT] = a1;
+          g[i + 2 * kT] = a2;
+          g[i + 3 * kT] = a3;
+        }
+      }
+      for (; i < words; i += kT) {
         if (to_lds)
           l[i] = g[i];
         else
@@ -466,6 +578,7 @@ struct acd_dev_lp {
   State* st;
   Prof* prof;
   double* partials;
+  uint32_t* ticket;  // arrivals of the fused step kernel's workgroups
   acd_event* ev;
   int blocks;
   hipGraphExec_t graph[kKeys];
@@ -565,6 +678,61 @@ void launch_step(int key, int blocks, hipStream_t st, const Args& a) {
   }
 }
 
+void launch_fused(int key, int blocks, hipStream_t st, const Args& a,
+                  uint32_t* ticket) {
+  const dim3 grid(blocks), block(64 * kWPB);
+  switch (key) {
+    case K_MIX_F32:
+      k_fused<double, float, float><<<grid, block, 0, st>>>(a, ticket);
+      break;
+    case K_MIX_BF16:
+      k_fused<double, float, uint16_t><<<grid, block, 0, st>>>(a, ticket);
+      break;
+    case K_F32:
+      k_fused<float, float, float><<<grid, block, 0, st>>>(a, ticket);
+      break;
+    case K_F32_BF16:
+      k_fused<float, float, uint16_t><<<grid, block, 0, st>>>(a, ticket);
+      break;
+    default:
+      k_fused<double, double, double><<<grid, block, 0, st>>>(a, ticket);
+      break;
+  }
+}
+
+// ACD_SOLVER_FUSED=1 runs each op as one fused launch instead of a step
+// kernel followed by a control kernel (same results bit for bit; slower
+// here, see k_fused).
+bool use_fused() {
+  static int on = -1;
+  if (on < 0) {
+    const char* e = getenv("ACD_SOLVER_FUSED");
+    on = e && e[0] == '1';
+  }
+  return on;
+}
+
+// One op: a fused launch, or a (step, control) pair.
+void launch_op(acd_dev_lp* lp, int key, hipStream_t st, const Args& a) {
+  if (use_fused()) {
+    launch_fused(key, lp->blocks, st, a, lp->ticket);
+  } else {
+    launch_step(key, lp->blocks, st, a);
+    k_ctl<<<1, 256, 0, st>>>(a, (uint32_t)lp->blocks);
+  }
+}
+
+inline uint64_t launches_per_op() { return use_fused() ? 1 : 2; }
+
+// The fused kernel runs nothing for OP_NONE; a live solve whose control
+// step issued no op (its continuation guard ran out) is advanced by a
+// control kernel before the next chunk.
+void kick_if_idle(acd_dev_lp* lp, const State* hs, hipStream_t st,
+                  const Args& a) {
+  if (use_fused() && hs->op == OP_NONE && hs->pc != PC_DONE && !hs->error)
+    k_ctl<<<1, 256, 0, st>>>(a, (uint32_t)lp->blocks);
+}
+
 int build_graph(acd_dev_lp* lp, int key) {
   if (lp->have_graph[key]) return 0;
   hipStream_t cs;
@@ -577,10 +745,7 @@ int build_graph(acd_dev_lp* lp, int key) {
       hipSuccess) {
     rc = -1;
   } else {
-    for (int i = 0; i < kPairs; i++) {
-      launch_step(key, lp->blocks, cs, a);
-      k_ctl<<<1, 256, 0, cs>>>(a, (uint32_t)lp->blocks);
-    }
+    for (int i = 0; i < kPairs; i++) launch_op(lp, key, cs, a);
     if (hipStreamEndCapture(cs, &g) != hipSuccess) rc = -1;
     if (!rc && hipGraphInstantiate(&lp->graph[key], g, nullptr, nullptr,
                                    0) != hipSuccess)
@@ -674,13 +839,10 @@ int run_to_completion(acd_dev_lp* lp, int key, State* hs,
       if (graphs) {
         if (hipGraphLaunch(lp->graph[key], S()) != hipSuccess) rc = -1;
       } else {
-        for (int i = 0; i < kPairs; i++) {
-          launch_step(key, lp->blocks, S(), a);
-          k_ctl<<<1, 256, 0, S()>>>(a, (uint32_t)lp->blocks);
-        }
+        for (int i = 0; i < kPairs; i++) launch_op(lp, key, S(), a);
         if (hipGetLastError() != hipSuccess) rc = -1;
       }
-      launches += 2 * kPairs;
+      launches += launches_per_op() * kPairs;
     }
     if (rc) {
       acd_dev_record_error(rc, "solver graph launch");
@@ -706,6 +868,7 @@ int run_to_completion(acd_dev_lp* lp, int key, State* hs,
                      hipMemcpyHostToDevice, S());
     }
     if (hs->error || hs->pc == PC_DONE) break;
+    kick_if_idle(lp, hs, S(), a);
     if (chunk < 64) chunk *= 2;
   }
   hipEventRecord(e1, S());
@@ -1041,6 +1204,14 @@ constexpr double kTuneMargin = 0.90;
 constexpr uint32_t kRoundLaunches = 8;
 constexpr uint64_t kBatchBudget = 32;
 
+// Iteration budget per launch (experiments: ACD_BATCH_BUDGET).
+uint64_t env_budget(const char* name, uint64_t dflt) {
+  const char* e = getenv(name);
+  if (!e) return dflt;
+  const long v = atol(e);
+  return v >= 1 && v <= (1 << 20) ? (uint64_t)v : dflt;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1093,10 +1264,11 @@ struct acd_dev_lp* acd_dev_lp_create(const struct acd_dev_matrix* orig,
   for (int i = 0; i < kNumVM; i++) lp->vm[i] = al(m);
   lp->blocks = pick_blocks(orig, perm);
   lp->partials = al((size_t)lp->blocks * kRes);
+  lp->ticket = (uint32_t*)acd_dev_malloc(sizeof(uint32_t));
   lp->st = (State*)acd_dev_malloc(sizeof(State));
   lp->prof = (Prof*)acd_dev_malloc(sizeof(Prof));
   lp->ev = (acd_event*)acd_dev_malloc(sizeof(acd_event) * kEventCap);
-  if (!ok || !lp->st || !lp->prof || !lp->ev) {
+  if (!ok || !lp->st || !lp->prof || !lp->ev || !lp->ticket) {
     acd_dev_lp_free(lp);
     return nullptr;
   }
@@ -1122,6 +1294,7 @@ void acd_dev_lp_free(struct acd_dev_lp* lp) {
   for (int i = 0; i < kNumVN; i++) acd_dev_free(lp->vn[i]);
   for (int i = 0; i < kNumVM; i++) acd_dev_free(lp->vm[i]);
   acd_dev_free(lp->partials);
+  acd_dev_free(lp->ticket);
   acd_dev_free(lp->st);
   acd_dev_free(lp->prof);
   acd_dev_free(lp->ev);
@@ -1298,6 +1471,7 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   uint32_t active = (uint32_t)count;
   uint32_t* live = h_slots;
   uint32_t done = 0;
+  const uint64_t budget = env_budget("ACD_BATCH_BUDGET", kBatchBudget);
   if (!rc &&
       upload_groups(live, active, choice, h_tmp, d_slots, grp, lds_bytes))
     rc = -1;
@@ -1306,7 +1480,7 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
     k_stamp<<<1, 1, 0, S()>>>(d_t0);
     for (int round = 0; !rc && active; round++) {
       const double t = now_s();
-      if (run_round(key, grp, kRoundLaunches, kBatchBudget, bufs, graphs,
+      if (run_round(key, grp, kRoundLaunches, budget, bufs, graphs,
                     true, r0) ||
           hipEventRecord(r1, S()) != hipSuccess ||
           hipMemcpyAsync(&done, d_count, sizeof done, hipMemcpyDeviceToHost,
@@ -1662,7 +1836,7 @@ int acd_dev_lp_async_begin(struct acd_dev_lp* lp,
                      st) != hipSuccess ||
       hipEventRecord(lp->aev, st) != hipSuccess)
     return -1;
-  lp->alaunches = 2 * kPairs + 1;
+  lp->alaunches = launches_per_op() * kPairs + 1;
   lp->abusy = 1;
   return 0;
 }
@@ -1691,9 +1865,10 @@ int acd_dev_lp_async_poll(struct acd_dev_lp* lp, struct acd_solve_result* r) {
     return 1;
   }
   hipStream_t st = lp->as;
+  kick_if_idle(lp, hs, st, make_args(lp, lp->akey));
   for (int c = 0; c < lp->achunk; c++)
     if (hipGraphLaunch(lp->graph[lp->akey], st) != hipSuccess) return -1;
-  lp->alaunches += (uint64_t)lp->achunk * 2 * kPairs;
+  lp->alaunches += (uint64_t)lp->achunk * launches_per_op() * kPairs;
   if (lp->achunk < 16) lp->achunk *= 2;
   if (hipMemcpyAsync(hs, lp->st, sizeof(State), hipMemcpyDeviceToHost, st) !=
           hipSuccess ||

@@ -104,7 +104,7 @@ const char * acd_pass_name(int p)
                 "finish", "permute_out", "spg_grad0", "spg_grad", "spg_step",
                 "alm_project", "alm_penalise", "permute_problem",
                 "step_sizes", "alm_violation", "alm_argmax", "alm_update",
-                "kkt", "pcd_step", "pcd_sizes"};
+                "kkt", "pcd_step", "pcd_sizes", "kkt_checkpoint"};
         if (p < 0 || p >= (int)(sizeof names / sizeof names[0])) return "";
         return names[p];
 }

@@ -68,7 +68,8 @@ enum Op : int32_t {
   OP_LAM_W,
   OP_KKT,
   OP_PCD_STEP,
-  OP_PCD_SIZES
+  OP_PCD_SIZES,
+  OP_KKT_CK
 };
 
 enum OpFlag : uint32_t {
@@ -94,7 +95,7 @@ enum Pc : int32_t {
   PC_A_INIT, PC_A_INIT1, PC_A_INIT2,
   PC_I0, PC_I_SHORT, PC_I_LONG1, PC_I_LONG2, PC_I_STEPPED, PC_I_FIX,
   PC_I_DD, PC_I_RESTARTED, PC_I_LIN2, PC_I_COMMIT, PC_I_DRIVER,
-  PC_I_CK1, PC_I_CK2, PC_I_CK3, PC_I_PRINT, PC_I_PRINT2, PC_I_NEXT,
+  PC_I_CK1, PC_I_CK2, PC_I_CK3, PC_I_CK4, PC_I_PRINT, PC_I_PRINT2, PC_I_NEXT,
   PC_F0, PC_F1, PC_F2, PC_F3,
   // SPG
   PC_M0, PC_M1, PC_M2, PC_M3, PC_MI, PC_MI2, PC_MI3, PC_MI5,
@@ -175,6 +176,7 @@ struct State {
   int32_t mixed;  // FP32 iterates: inner stall test at the FP32 floor
   double stop_kkt;  // > 0: also stop when the relative KKT residual <= it
   double kkt;       // last relative KKT residual (NaN until computed)
+  double kkt_cc, kkt_bb;  // sum of c_j^2 and of b_i^2 (set at ALM start)
   double alm_diag[3];
   int32_t alm_ret, inner_reason, upd, bump;
   double scale, argmax;
@@ -695,8 +697,12 @@ struct Ops {
       case OP_ALM_LAMPROJ: {  // alm.c:171-180, :314
         each(a.m, [&](uint32_t i) {
           a.lam[i] = fmax(a.lam_lo[i], fmin(a.lam[i], a.lam_hi[i]));
+          p[2] += a.rhs[i] * a.rhs[i];
         });
-        each(a.n, [&](uint32_t j) { p[0] += a.x_io[j] * a.alm_c[j]; });
+        each(a.n, [&](uint32_t j) {
+          p[0] += a.x_io[j] * a.alm_c[j];
+          p[1] += a.alm_c[j] * a.alm_c[j];
+        });
         break;
       }
       case OP_PENALISE: {  // alm.c:106-119
@@ -803,6 +809,44 @@ struct Ops {
         each(a.m, [&](uint32_t i) {
           p[4] += a.rhs[i] * a.lam[i];
           p[5] += a.rhs[i] * a.rhs[i];
+        });
+        break;
+      }
+      case OP_KKT_CK: {
+        // Relative KKT residual of (x, lambda') at an inner checkpoint of
+        // an ALM solve, lambda' = lambda - w (Ax - b) being the multipliers
+        // the outer step would set from this x (alm.c:182-192).  The
+        // checkpoint has just formed viol = w (Ax - b) and the gradient
+        // g = A' viol + c - A' lambda = c - A' lambda', so d = g needs no
+        // product; that holds while lambda' needs no projection onto the
+        // multiplier bounds, which slot 6 reports (max |projection|).
+        // Working-copy order; caller-order data through the index maps.
+        const VI* X = VN(NX);
+        const VI* Gv = VN(NG);
+        each(a.n, [&](uint32_t j) {
+          const uint32_t src = a.col_idx[j];
+          const double x = (double)X[j], d = (double)Gv[j];
+          const double lo = a.lower[src], up = a.upper[src];
+          const double pr = fmin(fmax(x - d, lo), up);
+          const double dp = fmax(d, 0.0), dn = fmax(-d, 0.0);
+          const bool lo_inf = !isfinite(lo), up_inf = !isfinite(up);
+          const double inf = (lo_inf ? dp : 0.0) + (up_inf ? dn : 0.0);
+          p[0] += (x - pr) * (x - pr);
+          p[1] += inf * inf;
+          p[2] += (lo_inf ? 0.0 : lo * dp) - (up_inf ? 0.0 : up * dn);
+          p[3] += a.alm_c[src] * x;
+        });
+        const VI* V = VM(MXV);
+        p[6] = 0.0;
+        each(a.m, [&](uint32_t i) {
+          const uint32_t src = a.row_idx[i];
+          const double v = (double)V[i];
+          const double r = v / a.p_w[i];
+          const double l1 = a.lam[src] - v;
+          const double l2 = fmax(a.lam_lo[src], fmin(l1, a.lam_hi[src]));
+          p[4] += a.p_rhs[i] * l2;
+          p[5] += r * r;
+          p[6] = fmax(p[6], fabs(l2 - l1));
         });
         break;
       }
@@ -971,7 +1015,7 @@ __device__ inline double alg_bytes(int32_t op, uint32_t fl, const Args& a) {
     case OP_MPG_GRAD0: return spT + m * si + n * (sc + 2 * si);
     case OP_MPG_GRAD: return spT + m * si + n * (3 * sc + 4 * si);
     case OP_MPG_ZUPD: return n * (3 * si + 2 * sc);
-    case OP_ALM_LAMPROJ: return 32 * m + 16 * n;
+    case OP_ALM_LAMPROJ: return 40 * m + 16 * n;
     case OP_PENALISE: return spT + 8 * m + 16 * n + 16 * m;
     case OP_PERMUTE_PROBLEM: return m * (40 + si) + n * (28 + 3 * sc);
     case OP_STEP_SIZES: return spT + m * si + 3 * n * sc;
@@ -981,6 +1025,7 @@ __device__ inline double alg_bytes(int32_t op, uint32_t fl, const Args& a) {
     case OP_KKT: return spT + 8 * m + 32 * n + 16 * m;
     case OP_PCD_STEP: return spT + m * si + n * (2 * si + 4 * sc);
     case OP_PCD_SIZES: return spT + m * 12 + n * (si + sc);
+    case OP_KKT_CK: return n * (2 * si + 4 + 24) + m * (si + 48);
     default: return 0;
   }
 }
@@ -1276,9 +1321,27 @@ struct Ctl {
         if (s.value < s.max_value) return stop(1);
         if (s.pg < s.max_pg) return stop(2);
         if (s.delta < s.min_delta) return stop(3);
+        if (s.mode == ACD_MODE_ALM && s.stop_kkt > 0) {
+          issue(OP_KKT_CK, 0, PC_I_CK4, 1u << 6);
+          break;
+        }
         s.prevx_pending = 1;
         s.pc = PC_I_PRINT;
         break;
+      case PC_I_CK4: {
+        // The ALM's KKT target is met at this x already: end the inner
+        // solve here, as if converged; the outer step then sets lambda'
+        // and confirms the residual with its own exact evaluation.
+        const double rp = sqrt(res[5]) / (1 + sqrt(s.kkt_bb));
+        const double rd = (sqrt(res[0]) + sqrt(res[1])) / (1 + sqrt(s.kkt_cc));
+        const double dual = res[4] + res[2];
+        const double rg = fabs(res[3] - dual) / (1 + fabs(res[3]) + fabs(dual));
+        const double k = fmax(rp, fmax(rd, rg));
+        if (res[6] == 0.0 && k <= s.stop_kkt) return stop(2);
+        s.prevx_pending = 1;
+        s.pc = PC_I_PRINT;
+        break;
+      }
       case PC_I_PRINT:  // approx/approx.c:666-674
         if (s.i == 0 || (s.period && (s.i + 1) % s.period == 0)) {
           if (s.restart_flag) {
@@ -1338,6 +1401,8 @@ struct Ctl {
         break;
       case PC_L1:
         s.cx = res[0];
+        s.kkt_cc = res[1];
+        s.kkt_bb = res[2];
         s.outer = 0;
         s.precision = 1;
         s.prev_norm = HUGE_VAL;

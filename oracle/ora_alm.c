@@ -243,6 +243,54 @@ double ora_alm_kkt(ora_alm_t * a, const double * x, const double * lambda)
         return fmax(rp, fmax(rd, rg));
 }
 
+/* The device library's inner-checkpoint test for its KKT target (the
+ * OP_KKT_CK op and its continuation): the relative KKT residual of
+ * (x, lambda'), lambda' = lambda - w (Ax - b), from the checkpoint's
+ * violation and gradient g = c - A' lambda' (no product needed), valid
+ * while lambda' needs no projection onto the multiplier bounds.  Sums of
+ * c^2 and b^2 are taken once per solve, in the caller's order. */
+struct kkt_ckpt {
+        ora_alm_t * alm;
+        const double * lambda;      /* caller order, current */
+        double stop, cc, bb;
+        double * c, * lam, * lam_lo, * lam_hi;  /* working order */
+};
+
+static int kkt_checkpoint(void * ctx, const struct ora_approx * ap,
+                          const double * x, const double * viol,
+                          const double * g)
+{
+        struct kkt_ckpt * k = ctx;
+        const size_t n = ap->nvars, m = ap->nrhs;
+        ora_sparse_matrix_row_permute(ap->matrix, k->lam, m, k->lambda, 1);
+        double px = 0, inf2 = 0, bnd = 0, cx = 0;
+        for (size_t j = 0; j < n; j++) {
+                const double d = g[j], lo = ap->lower[j], up = ap->upper[j];
+                const double pr = fmin(fmax(x[j] - d, lo), up);
+                const double dp = fmax(d, 0), dn = fmax(-d, 0);
+                const int lo_inf = !isfinite(lo), up_inf = !isfinite(up);
+                const double inf = (lo_inf ? dp : 0) + (up_inf ? dn : 0);
+                px += (x[j] - pr) * (x[j] - pr);
+                inf2 += inf * inf;
+                bnd += (lo_inf ? 0 : lo * dp) - (up_inf ? 0 : up * dn);
+                cx += k->c[j] * x[j];
+        }
+        double bl = 0, rr = 0, proj = 0;
+        for (size_t i = 0; i < m; i++) {
+                const double r = viol[i] / ap->weight[i];
+                const double l1 = k->lam[i] - viol[i];
+                const double l2 = fmax(k->lam_lo[i], fmin(l1, k->lam_hi[i]));
+                bl += ap->rhs[i] * l2;
+                rr += r * r;
+                proj = fmax(proj, fabs(l2 - l1));
+        }
+        const double dual = bl + bnd;
+        const double rp = sqrt(rr) / (1 + sqrt(k->bb));
+        const double rd = (sqrt(px) + sqrt(inf2)) / (1 + sqrt(k->cc));
+        const double rg = fabs(cx - dual) / (1 + fabs(cx) + fabs(dual));
+        return proj == 0 && fmax(rp, fmax(rd, rg)) <= k->stop;
+}
+
 /* alm.c:289-322 */
 int ora_alm_solve(ora_alm_t * a, size_t niter, double * x, size_t nvars,
                   double * lambda, size_t ncons, FILE * log, double * diag,
@@ -259,6 +307,27 @@ int ora_alm_solve(ora_alm_t * a, size_t niter, double * x, size_t nvars,
         project_multipliers(lambda, a);
         double pg = HUGE_VAL, max_viol = HUGE_VAL;
         int ret = 1;
+        struct kkt_ckpt ck = {a, lambda, o.stop_kkt, 0, 0, NULL, NULL, NULL,
+                              NULL};
+        if (o.stop_kkt > 0) {
+                const ora_approx_t * ap = a->approx->permuted
+                        ? a->approx->permuted : a->approx;
+                const size_t n = a->nvars, m = a->nrhs;
+                for (size_t j = 0; j < n; j++)
+                        ck.cc += a->linear[j] * a->linear[j];
+                for (size_t i = 0; i < m; i++)
+                        ck.bb += a->approx->rhs[i] * a->approx->rhs[i];
+                ck.c = calloc(n + 1, sizeof(double));
+                ck.lam = calloc(m + 1, sizeof(double));
+                ck.lam_lo = calloc(m + 1, sizeof(double));
+                ck.lam_hi = calloc(m + 1, sizeof(double));
+                ora_sparse_matrix_col_permute(ap->matrix, ck.c, n, a->linear, 1);
+                ora_sparse_matrix_row_permute(ap->matrix, ck.lam_lo, m,
+                                              a->lambda_lower, 1);
+                ora_sparse_matrix_row_permute(ap->matrix, ck.lam_hi, m,
+                                              a->lambda_upper, 1);
+                ora_approx_set_ckpt_hook(kkt_checkpoint, &ck);
+        }
         for (size_t i = 0; i < niter; i++) {
                 int status = iter(&s, a, x, lambda, log, i + 1, &pg,
                                   &max_viol, &o);
@@ -278,6 +347,13 @@ int ora_alm_solve(ora_alm_t * a, size_t niter, double * x, size_t nvars,
                 diag[0] = seq_dot(x, a->linear, a->nvars);
                 diag[1] = max_viol;
                 diag[2] = pg;
+        }
+        if (o.stop_kkt > 0) {
+                ora_approx_set_ckpt_hook(NULL, NULL);
+                free(ck.c);
+                free(ck.lam);
+                free(ck.lam_lo);
+                free(ck.lam_hi);
         }
         free(s.viol);
         return ret;

@@ -492,6 +492,15 @@ static void print_log(FILE * log, size_t k, double value, double ng,
 }
 
 /* approx/approx.c:576-708 */
+static ora_ckpt_hook ckpt_hook;
+static void * ckpt_ctx;
+
+void ora_approx_set_ckpt_hook(ora_ckpt_hook fn, void * ctx)
+{
+        ckpt_hook = fn;
+        ckpt_ctx = ctx;
+}
+
 int ora_approx_solve(double * xio, size_t n, ora_approx_t * a, size_t niter,
                      double max_pg, double max_value, double min_delta,
                      FILE * log, size_t period, double * diag, double offset)
@@ -543,6 +552,11 @@ int ora_approx_solve(double * xio, size_t n, ora_approx_t * a, size_t niter,
                         if (value < max_value) { reason = 1; break; }
                         if (pg < max_pg) { reason = 2; break; }
                         if (delta < min_delta) { reason = 3; break; }
+                        if (ckpt_hook != NULL &&
+                            ckpt_hook(ckpt_ctx, a, s.x.x, s.x.viol, s.g.x)) {
+                                reason = 2;
+                                break;
+                        }
                         memcpy(prev_x, s.x.x, n * sizeof(double));
                 }
                 if ((i == 0) || (period && ((i + 1) % period == 0))) {

@@ -14,4 +14,13 @@ struct ora_approx {
         struct ora_approx * permuted;
 };
 
+/* Called at every checkpoint of ora_approx_solve (every 100 iterations,
+ * after x, its violation w (Ax - b) and its gradient are formed, all in
+ * the working copy's order) unless NULL; a nonzero return ends the solve
+ * as converged (reason 2).  ora_alm_solve sets it for its KKT target. */
+typedef int (*ora_ckpt_hook)(void * ctx, const struct ora_approx * a,
+                             const double * x, const double * viol,
+                             const double * g);
+void ora_approx_set_ckpt_hook(ora_ckpt_hook fn, void * ctx);
+
 #endif

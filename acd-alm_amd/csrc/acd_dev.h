@@ -58,6 +58,8 @@ struct acd_sell {
         double * val;          /* device [nent] */
         float * val32;         /* device [nent] or NULL (mixed precision) */
         uint16_t * val16;      /* device [nent] bf16 bits, or NULL */
+        uint32_t * colsgn;     /* device [nent] column | sign bits when every
+                                  value is -1, 0 or +1 (value-free), or NULL */
         uint32_t * rowlen;     /* device [nrows]: entries per row */
         int32_t tuned;         /* SpMV variant + 1 picked by the autotuner,
                                   0 = library default */
@@ -90,6 +92,11 @@ int acd_dev_sell_make_fp32(struct acd_sell *);
  * representable (0/+-1 and small dyadic coefficients).  Returns 1 when the
  * copy exists, 0 when some value is not exact (no copy), -1 on error. */
 int acd_dev_sell_make_bf16(struct acd_sell *);
+/* Value-free copy of a 0/+-1 operator: each stored entry as one word, the
+ * column in bits 0-29, bit 31 for a value of -1, bit 30 for 0 (padding).
+ * Returns 1 when made, 0 when some value is not -1, 0 or +1 or some
+ * column does not fit in 30 bits (no copy), -1 on error. */
+int acd_dev_sell_make_sign(struct acd_sell *);
 
 /* ---- vector kernels (dev_spmv.hip) ---- */
 /* y = op x on device pointers (op = A' when transpose). */

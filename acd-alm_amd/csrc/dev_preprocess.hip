@@ -185,6 +185,25 @@ __global__ void k_to_bf16(const double* src, uint16_t* dst, uint32_t* inexact,
   if ((double)f != d || (bits & 0xffffu)) atomicOr(inexact, 1u);
 }
 
+// Value-free entry words (acd_dev_sell_make_sign): flags any value other
+// than -1, 0 or +1 and any column of 30 bits or more.
+__global__ void k_to_sign(const double* val, const uint32_t* col,
+                          uint32_t* dst, uint32_t* inexact, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double d = val[i];
+  const uint32_t c = col[i];
+  uint32_t w = c & 0x3fffffffu;
+  if (d == -1.0)
+    w |= 0x80000000u;
+  else if (d == 0.0)
+    w |= 0x40000000u;
+  else if (d != 1.0)
+    atomicOr(inexact, 1u);
+  if (c > 0x3fffffffu) atomicOr(inexact, 1u);
+  dst[i] = w;
+}
+
 struct Scratch {
   void* p = nullptr;
   size_t bytes = 0;
@@ -341,6 +360,7 @@ void free_sell(struct acd_sell* op) {
   hipFree(op->val);
   hipFree(op->val32);
   hipFree(op->val16);
+  hipFree(op->colsgn);
   hipFree(op->rowlen);
   memset(op, 0, sizeof *op);
 }
@@ -529,6 +549,38 @@ int acd_dev_sell_make_bf16(struct acd_sell* op) {
     return 0;
   }
   op->val16 = v16;
+  return 1;
+}
+
+int acd_dev_sell_make_sign(struct acd_sell* op) {
+  if (op->colsgn) return 1;
+  uint32_t* w = nullptr;
+  uint32_t* inexact = nullptr;
+  CK(hipMalloc((void**)&w, sizeof(uint32_t) * (op->nent ? op->nent : 1)),
+     "hipMalloc value-free entries");
+  if (hipMalloc((void**)&inexact, sizeof(uint32_t)) != hipSuccess) {
+    hipFree(w);
+    acd_dev_record_error(-1, "hipMalloc value-free flag");
+    return -1;
+  }
+  uint32_t h = 0;
+  hipMemsetAsync(inexact, 0, sizeof(uint32_t), S());
+  if (op->nent)
+    k_to_sign<<<blocks_for(op->nent, 256), 256, 0, S()>>>(op->val, op->col, w,
+                                                          inexact, op->nent);
+  hipMemcpyAsync(&h, inexact, sizeof h, hipMemcpyDeviceToHost, S());
+  if (hipStreamSynchronize(S()) != hipSuccess) {
+    hipFree(w);
+    hipFree(inexact);
+    acd_dev_record_error(-1, "value-free conversion");
+    return -1;
+  }
+  hipFree(inexact);
+  if (h) {
+    hipFree(w);
+    return 0;
+  }
+  op->colsgn = w;
   return 1;
 }
 

@@ -34,6 +34,7 @@ enum {
   K_MIX_BF16,   // double / float / bf16
   K_F32,        // float / float / float
   K_F32_BF16,   // float / float / bf16
+  K_MIX_SGN,    // double / float / value-free (0/+-1 operators)
   kKeys
 };
 uint64_t g_prof_count[32];
@@ -559,6 +560,7 @@ Sell view(const acd_sell& op) {
   v.val = op.val;
   v.val32 = op.val32;
   v.val16 = op.val16;
+  v.colsgn = op.colsgn;
   v.nrows = op.nrows;
   v.nslices = op.nslices;
   return v;
@@ -596,7 +598,7 @@ namespace {
 
 Args make_args(acd_dev_lp* lp, int key) {
   static const uint32_t kSizes[kKeys][3] = {
-      {8, 8, 8}, {8, 4, 4}, {8, 4, 2}, {4, 4, 4}, {4, 4, 2}};
+      {8, 8, 8}, {8, 4, 4}, {8, 4, 2}, {4, 4, 4}, {4, 4, 2}, {8, 4, 0}};
   Args a;
   memset(&a, 0, sizeof a);
   a.m = lp->m;
@@ -674,6 +676,7 @@ void launch_step(int key, int blocks, hipStream_t st, const Args& a) {
     case K_F32_BF16:
       k_step<float, float, uint16_t><<<grid, block, 0, st>>>(a);
       break;
+    case K_MIX_SGN: k_step<double, float, SgnT><<<grid, block, 0, st>>>(a); break;
     default: k_step<double, double, double><<<grid, block, 0, st>>>(a); break;
   }
 }
@@ -693,6 +696,9 @@ void launch_fused(int key, int blocks, hipStream_t st, const Args& a,
       break;
     case K_F32_BF16:
       k_fused<float, float, uint16_t><<<grid, block, 0, st>>>(a, ticket);
+      break;
+    case K_MIX_SGN:
+      k_fused<double, float, SgnT><<<grid, block, 0, st>>>(a, ticket);
       break;
     default:
       k_fused<double, double, double><<<grid, block, 0, st>>>(a, ticket);
@@ -933,6 +939,19 @@ int precision_key(acd_dev_lp* lp, int precision) {
     for (acd_sell* op : ops)
       if (acd_dev_sell_make_fp32(op)) return -1;
   if (precision == ACD_DEV_FP32) return bf ? K_F32_BF16 : K_F32;
+  // Operators whose values are all -1, 0 or +1 (set cover, network flow)
+  // drop the value stream altogether; the products are the BF16 form's
+  // bit for bit (ACD_VALUE_FREE=0 keeps the BF16 form).
+  const char* vf = getenv("ACD_VALUE_FREE");
+  if (bf && !(vf && vf[0] == '0')) {
+    bool sg = true;
+    for (acd_sell* op : ops) {
+      const int r = acd_dev_sell_make_sign(op);
+      if (r < 0) return -1;
+      sg = sg && r == 1;
+    }
+    if (sg) return K_MIX_SGN;
+  }
   return bf ? K_MIX_BF16 : K_MIX_F32;
 }
 
@@ -1203,13 +1222,25 @@ constexpr size_t kTuneMinLps = 192;
 constexpr double kTuneMargin = 0.90;
 constexpr uint32_t kRoundLaunches = 8;
 constexpr uint64_t kBatchBudget = 32;
+// Once the workgroup sizes are chosen, a batch runs every LP to
+// completion in its workgroup: the hardware deals the workgroups out in
+// launch order (largest LPs first) as slots free up, so every slot stays
+// busy until the last LPs start, and an LP that needs many iterations
+// runs at the speed of a resident workgroup instead of advancing 32
+// iterations per round of the whole batch.  Measured on 4,096 LPs: 612
+// against 487 solves/s with 32-iteration rounds (4,096 iterations: 505).
+// The results do not depend on the budget (the state is saved exactly).
+// 65,536 iterations cover every LP of that batch (at most 60,370) and
+// bound one workgroup's share of a launch to a few seconds.
+constexpr uint64_t kRunBudget = 1ull << 16;
 
-// Iteration budget per launch (experiments: ACD_BATCH_BUDGET).
-uint64_t env_budget(const char* name, uint64_t dflt) {
-  const char* e = getenv(name);
-  if (!e) return dflt;
+// Iteration budget per launch after tuning: kRunBudget, or a fixed one
+// for every round (experiments: ACD_BATCH_BUDGET).
+uint64_t run_budget() {
+  const char* e = getenv("ACD_BATCH_BUDGET");
+  if (!e) return kRunBudget;
   const long v = atol(e);
-  return v >= 1 && v <= (1 << 20) ? (uint64_t)v : dflt;
+  return v >= 1 ? (uint64_t)v : kRunBudget;
 }
 
 }  // namespace
@@ -1359,7 +1390,7 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
     for (size_t i = 0; i < count && all_bf; i++) {
       const int k = precision_key(lps[i], q->precision);
       if (k < 0) return -1;
-      all_bf = k == K_MIX_BF16 || k == K_F32_BF16;
+      all_bf = k == K_MIX_BF16 || k == K_F32_BF16 || k == K_MIX_SGN;
     }
     if (!all_bf)
       for (size_t i = 0; i < count; i++) {
@@ -1466,12 +1497,16 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
   Group grp[kWpbChoices] = {};
   const bool graphs = use_graphs();
   // Relaunch over the unfinished LPs (compacted between rounds) until all
-  // are done; the budget per launch is 32 inner iterations, 8 launches per
-  // group per round (one captured graph per group while its list holds).
+  // are done, 8 launches per group per round (one captured graph per group
+  // while its list holds): 32 inner iterations per launch while the tuner
+  // compares workgroup sizes, then run to completion.
   uint32_t active = (uint32_t)count;
   uint32_t* live = h_slots;
   uint32_t done = 0;
-  const uint64_t budget = env_budget("ACD_BATCH_BUDGET", kBatchBudget);
+  const char* fixed_budget = getenv("ACD_BATCH_BUDGET");
+  const uint64_t after_tuning = run_budget();
+  uint64_t budget =
+      trial >= 0 && !fixed_budget ? kBatchBudget : after_tuning;
   if (!rc &&
       upload_groups(live, active, choice, h_tmp, d_slots, grp, lds_bytes))
     rc = -1;
@@ -1553,6 +1588,9 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
           choice[lp] = (uint8_t)lp_choice(cls[lp], forced, cfg);
         }
         active = k;
+        // A new budget is captured with the new lists (upload_groups
+        // drops the graphs).
+        budget = trial >= 0 && !fixed_budget ? kBatchBudget : after_tuning;
         if (upload_groups(live, active, choice, h_tmp, d_slots, grp,
                           lds_bytes)) {
           rc = -1;

@@ -128,8 +128,24 @@ struct Sell {
   const double* val;       // FP64 values
   const float* val32;      // FP32 copy (mixed precision) or NULL
   const uint16_t* val16;   // BF16 copy when every value is exact, or NULL
+  const uint32_t* colsgn;  // value-free entries of a 0/+-1 operator, or NULL
   uint32_t nrows, nslices;
 };
+
+// Matrix-value type of a value-free operator: every stored value is -1, 0
+// or +1 and travels in the column word (bit 31: -1, bit 30: 0), so an
+// entry costs 4 bytes instead of 6 (BF16) or 8 (FP32).
+struct SgnT {};
+
+// Column of entry e as stored for precision MT.
+template <class MT>
+__device__ __forceinline__ uint32_t mcol(const Sell& M, uint64_t e) {
+  return M.col[e];
+}
+template <>
+__device__ __forceinline__ uint32_t mcol<SgnT>(const Sell& M, uint64_t e) {
+  return M.colsgn[e] & 0x3fffffffu;
+}
 
 // Value of entry e as stored for precision MT (double, float, or bf16
 // carried in a uint16_t), widened to double for the FP64 accumulation.
@@ -146,6 +162,11 @@ __device__ __forceinline__ double mval<float>(const Sell& M, uint64_t e) {
 template <>
 __device__ __forceinline__ double mval<uint16_t>(const Sell& M, uint64_t e) {
   return (double)__uint_as_float((uint32_t)M.val16[e] << 16);
+}
+template <>
+__device__ __forceinline__ double mval<SgnT>(const Sell& M, uint64_t e) {
+  const uint32_t w = M.colsgn[e];
+  return (w & 0x40000000u) ? 0.0 : ((w & 0x80000000u) ? -1.0 : 1.0);
 }
 
 struct State {
@@ -322,8 +343,8 @@ struct Ops {
       double a0 = 0.0, a1 = 0.0;
       uint32_t k = 0;
       if (len >= 4) {
-        uint32_t c0 = M.col[e], c1 = M.col[e + 64], c2 = M.col[e + 128],
-                 c3 = M.col[e + 192];
+        uint32_t c0 = mcol<MT2>(M, e), c1 = mcol<MT2>(M, e + 64),
+                 c2 = mcol<MT2>(M, e + 128), c3 = mcol<MT2>(M, e + 192);
         double v0 = mval<MT2>(M, e), v1 = mval<MT2>(M, e + 64),
                v2 = mval<MT2>(M, e + 128), v3 = mval<MT2>(M, e + 192);
         for (;;) {
@@ -332,10 +353,10 @@ struct Ops {
           double w0 = 0, w1 = 0, w2 = 0, w3 = 0;
           if (more) {
             const uint64_t en = e + 256;
-            d0 = M.col[en];
-            d1 = M.col[en + 64];
-            d2 = M.col[en + 128];
-            d3 = M.col[en + 192];
+            d0 = mcol<MT2>(M, en);
+            d1 = mcol<MT2>(M, en + 64);
+            d2 = mcol<MT2>(M, en + 128);
+            d3 = mcol<MT2>(M, en + 192);
             w0 = mval<MT2>(M, en);
             w1 = mval<MT2>(M, en + 64);
             w2 = mval<MT2>(M, en + 128);
@@ -365,9 +386,9 @@ struct Ops {
       const uint32_t rem = len - k;
       if (rem) {
         const bool h1 = rem > 1, h2 = rem > 2;
-        const uint32_t c0 = M.col[e];
-        const uint32_t c1 = h1 ? M.col[e + 64] : 0u;
-        const uint32_t c2 = h2 ? M.col[e + 128] : 0u;
+        const uint32_t c0 = mcol<MT2>(M, e);
+        const uint32_t c1 = h1 ? mcol<MT2>(M, e + 64) : 0u;
+        const uint32_t c2 = h2 ? mcol<MT2>(M, e + 128) : 0u;
         const double v0 = mval<MT2>(M, e);
         const double v1 = h1 ? mval<MT2>(M, e + 64) : 0.0;
         const double v2 = h2 ? mval<MT2>(M, e + 128) : 0.0;
@@ -746,7 +767,7 @@ struct Ops {
           for (uint32_t k = 0; k < len; k++) {
             const uint64_t e = base + (uint64_t)k * 64 + g.lane;
             const double val = mval<MT>(M, e);
-            acc = acc + ((double)wb[M.col[e]] * val) * val;
+            acc = acc + ((double)wb[mcol<MT>(M, e)] * val) * val;
           }
           const uint32_t j = sl * 64 + g.lane;
           if (j < M.nrows) {
@@ -925,7 +946,7 @@ struct Ops {
           double acc = 0.0;
           for (uint32_t k = 0; k < len; k++) {
             const uint64_t e = base + (uint64_t)k * 64 + g.lane;
-            acc = acc + mval<MT>(M, e) * (double)sv[M.col[e]];
+            acc = acc + mval<MT>(M, e) * (double)sv[mcol<MT>(M, e)];
           }
           const double gj = acc + (double)pc[j];
           X[j] = (VI)clamp_lu(j, (double)X[j] - gj * (double)ist[j]);
@@ -947,7 +968,7 @@ struct Ops {
           double acc = 0.0;
           for (uint32_t k = 0; k < len; k++) {
             const uint64_t e = base + (uint64_t)k * 64 + g.lane;
-            const uint32_t i = M.col[e];
+            const uint32_t i = mcol<MT>(M, e);
             const double val = mval<MT>(M, e);
             const double wi =
                 (double)a.p_w[i] * (1.0 + pr * ((double)a.beta[i] - 1.0));

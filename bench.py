@@ -292,7 +292,13 @@ def run_batch(args, lib):
         wall = time.perf_counter() - t
         bad = [i for i, r in enumerate(out) if r[0] != 0]
         if bad:
-            raise RuntimeError("batch: %d LPs did not converge" % len(bad))
+            raise RuntimeError(
+                "batch: %d LPs did not converge: %s" % (len(bad), "; ".join(
+                    "LP %d (m=%d n=%d): ret %d, objective %.12g, max "
+                    "violation %.3g, %d inner iterations" % (
+                        i, lps[i].nrows, lps[i].ncols, out[i][0],
+                        out[i][3][0], out[i][3][1], int(info.inner[i]))
+                    for i in bad[:4])))
         return out, done, info, wall
 
     for w in range(args.warmup):

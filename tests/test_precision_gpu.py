@@ -111,3 +111,34 @@ def test_kkt_stopping_rule(gpu, name):
     assert rc == 0
     assert np.isfinite(dev) and dev <= 1e-6
     assert abs(dev - host) <= 1e-6 * host + 1e-13, (dev, host)
+
+
+@pytest.mark.parametrize("name", ["setcover", "mcf"])
+def test_value_free_operator_equals_bf16(gpu, name, monkeypatch):
+    """0/+-1 operators in mixed precision drop their value stream (the sign
+    travels in the column word): the multi-workgroup solve is the BF16
+    form's bit for bit, and each A pass moves 2 bytes per nonzero less."""
+    lp = {"setcover": I.set_cover(3000, 12000, 5),
+          "mcf": I.multicommodity(8, 4, 3)}[name]
+    print(f"[precision] value-free {name}", flush=True)
+    m = gpu.make_matrix(lp, 0)
+    alm = gpu.alm_make(m, lp)
+    out = {}
+    for form in ("bf16", "value-free"):
+        if form == "bf16":
+            monkeypatch.setenv("ACD_VALUE_FREE", "0")
+        else:
+            monkeypatch.delenv("ACD_VALUE_FREE")
+        gpu.lib.acd_alm_reset(alm)
+        r = gpu.alm_solve_ex(alm, 1000, np.zeros(lp.ncols), np.zeros(lp.nrows),
+                             precision=1, engine=1, stop_kkt=1e-6)
+        cnt, _, by = gpu.profile_bytes()["pass2"]
+        out[form] = (r, by / cnt)
+    gpu.alm_free(alm)
+    gpu.free_matrix(m)
+    (rb, bb), (rv, bv) = out["bf16"], out["value-free"]
+    assert rb[0] == rv[0] == 0
+    assert np.array_equal(rb[1], rv[1]) and np.array_equal(rb[2], rv[2])
+    assert np.array_equal(rb[3], rv[3])
+    assert bb - bv == 2 * lp.nnz, (bb, bv, lp.nnz)
+    assert kkt(lp, rv[1], rv[2])["kkt"] <= 1e-6

@@ -174,7 +174,7 @@ __device__ __forceinline__ void control_block(const Args& a, uint32_t nblocks,
   __syncthreads();
   if (threadIdx.x == 0 && live) {
     const uint64_t t_comb = wall_clock64();
-    Ctl c{ls, a, res};
+    Ctl c{*(LdsState*)&ls, a, (const LdsDouble*)res};
     c.advance<OutOfLine>();
     ls.t_last = wall_clock64();
     if (ran) {
@@ -316,10 +316,13 @@ extern __shared__ __align__(16) unsigned char lds_vecs[];
 
 // Copies the resident slots between their global homes and LDS
 // (to_lds) or back; slot order as assigned by k_batch.  Each slot is
-// copied in 16-byte words, four per thread in flight: a slot's LDS size
-// is its length rounded up to 16 bytes, which the global vector's spare
-// element covers (vectors hold length + 1 doubles), and the padding word
-// goes back unchanged (ops write only indices below the length).
+// copied in 16-byte words: a slot's LDS size is its length rounded up to
+// 16 bytes, which the global vector's spare element covers (vectors hold
+// length + 1 doubles), and the padding word goes back unchanged (ops
+// write only indices below the length).  The loop is kept plain: every
+// byte of the kernel's code competes for the instruction cache that two
+// CUs share (unrolled 4 ways it added 3.9 KB), and with LPs run to
+// completion the copies happen once per LP.
 template <int WPB>
 __device__ void lds_copy(const Args& la, const State& ls, uint32_t resident,
                          void* const* home, bool to_lds) {
@@ -335,25 +338,7 @@ __device__ void lds_copy(const Args& la, const State& ls, uint32_t resident,
     for (int e = 0; e < 2; e++) {
       uint4* g = reinterpret_cast<uint4*>(home[k]);
       uint4* l = reinterpret_cast<uint4*>(lds_vecs + off);
-      uint32_t i = threadIdx.x;
-      for (; i + 3 * kT < words; i += 4 * kT) {
-        if (to_lds) {
-          const uint4 a0 = g[i], a1 = g[i + kT], a2 = g[i + 2 * kT],
-                      a3 = g[i + 3 * kT];
-          l[i] = a0;
-          l[i + kT] = a1;
-          l[i + 2 * kT] = a2;
-          l[i + 3 * kT] = a3;
-        } else {
-          const uint4 a0 = l[i], a1 = l[i + kT], a2 = l[i + 2 * kT],
-                      a3 = l[i + 3 * kT];
-          g[i] = a0;
-          g[i + kT] = a1;
-          g[i + 2 * kT] = a2;
-          g[i + 3 * kT] = a3;
-        }
-      }
-      for (; i < words; i += kT) {
+      for (uint32_t i = threadIdx.x; i < words; i += kT) {
         if (to_lds)
           l[i] = g[i];
         else
@@ -479,7 +464,7 @@ k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
       const uint64_t t1 = wall_clock64();
       t_ops += t1 - t_mark;
       if (ls.op != OP_NONE) moved += (uint64_t)alg_bytes(ls.op, ls.op_flags, la);
-      Ctl c{ls, la, res};
+      Ctl c{*(LdsState*)&ls, la, (const LdsDouble*)res};
       c.advance<kBatchCtlCall, true>();
       t_mark = wall_clock64();
       t_ctl += t_mark - t1;

@@ -1053,10 +1053,16 @@ __device__ inline double alg_bytes(int32_t op, uint32_t fl, const Args& a) {
 
 // ---------------------------------------------------------------------------
 // Control: consumes the reduced results of the op just run, advances.
+// The control step always works on the state record and the reduced
+// results in LDS; saying so in the types lets an out-of-line continuation
+// address them with LDS instructions instead of generic ones.
+typedef __attribute__((address_space(3))) State LdsState;
+typedef __attribute__((address_space(3))) double LdsDouble;
+
 struct Ctl {
-  State& s;
+  LdsState& s;
   const Args& a;
-  const double* res;
+  const LdsDouble* res;
 
   __device__ void emit(int32_t type, uint64_t k, double a0 = 0, double a1 = 0,
                        double a2 = 0, double a3 = 0, double a4 = 0,
@@ -1093,7 +1099,7 @@ struct Ctl {
 
   template <class T>
   __device__ static void swap_(T& x, T& y) {
-    T t = x;
+    const auto t = +x;  // the value, without the LDS qualifier of x
     x = y;
     y = t;
   }

@@ -492,8 +492,10 @@ static void print_log(FILE * log, size_t k, double value, double ng,
 }
 
 /* approx/approx.c:576-708 */
-static ora_ckpt_hook ckpt_hook;
-static void * ckpt_ctx;
+/* Per thread: bench's CPU baseline solves several LPs at once, one per
+ * thread. */
+static __thread ora_ckpt_hook ckpt_hook;
+static __thread void * ckpt_ctx;
 
 void ora_approx_set_ckpt_hook(ora_ckpt_hook fn, void * ctx)
 {

@@ -17,7 +17,8 @@ struct ora_approx {
 /* Called at every checkpoint of ora_approx_solve (every 100 iterations,
  * after x, its violation w (Ax - b) and its gradient are formed, all in
  * the working copy's order) unless NULL; a nonzero return ends the solve
- * as converged (reason 2).  ora_alm_solve sets it for its KKT target. */
+ * as converged (reason 2).  ora_alm_solve sets it for its KKT target, for
+ * the calling thread only. */
 typedef int (*ora_ckpt_hook)(void * ctx, const struct ora_approx * a,
                              const double * x, const double * viol,
                              const double * g);

@@ -279,3 +279,34 @@ def test_warm_start_equals_the_reference_build(oracle):
     oracle.alm_free(lo)
     ref.free_matrix(mr)
     oracle.free_matrix(mo)
+
+
+def test_threaded_kkt_solves_match_serial(oracle, tmp_path):
+    """The CPU baseline (ora_bench alm) solves several LPs at once, one per
+    thread, each stopping on the KKT target with the inner-checkpoint test:
+    every LP meets the target and ends where the same solve on one thread
+    (the Python binding) ends."""
+    import json
+    import subprocess
+    import pyoracle
+    bench = os.path.join(os.path.dirname(pyoracle.ORACLE_SO), "ora_bench")
+    lps = I.batch(12, seed=11, mmin=30, mmax=150)
+    files = []
+    for k, lp in enumerate(lps):
+        f = str(tmp_path / f"lp{k}.lpb")
+        lp.write_binary(f)
+        files.append(f)
+    r = subprocess.run([bench, "alm", "4", "1e-6"] + files, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-500:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert max(out["kkt"]) <= 1e-6
+    for lp, obj in zip(lps, out["objective"]):
+        m = oracle.make_matrix(lp, 0)
+        a = oracle.alm_make(m, lp)
+        rc, x, lam, diag, _ = oracle.alm_solve(a, 1000, np.zeros(lp.ncols),
+                                               np.zeros(lp.nrows), stop_kkt=1e-6)
+        oracle.alm_free(a)
+        oracle.free_matrix(m)
+        assert rc == 0
+        assert diag[0] == obj

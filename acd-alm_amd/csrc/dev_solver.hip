@@ -284,13 +284,15 @@ struct BlockG {
   uint32_t lane, gwave, nwaves;
 };
 
-// The control continuation is called out of line (Ctl::advance<true>):
-// inlined, its temporaries took the kernel to ~240 VGPRs (2 waves per
-// SIMD); called, the kernel needs ~150 and runs 3 waves per SIMD, i.e. 3
-// LP workgroups of 4 waves per CU instead of 2 (measured on 4,096 LPs:
-// 344 against 336 solves/s; bounding it to 4 waves per SIMD spills and
-// ran 316).
-constexpr bool kBatchCtlCall = true;
+// The control continuation is inlined, with the kernel held to 3 waves per
+// SIMD (amdgpu_waves_per_eu(3): 168 VGPRs, a few spilled), i.e. 3 LP
+// workgroups of 4 waves per CU.  Called out of line (Ctl::advance<true>)
+// the kernel needs ~150 VGPRs without spills, but every control step pays
+// the call, its register saves in scratch and the reloads of the Ctl
+// members: measured on 4,096 LPs run to completion, 621 inlined against
+// 604 solves/s called.  (With 32-iteration rounds and no register bound,
+// inlining took ~240 VGPRs, 2 waves per SIMD: 336 against 344.)
+constexpr bool kBatchCtlCall = false;
 
 // Vector pairs that a batched LP may keep in LDS, in the order they are
 // given LDS (first the candidate point z / zp that the A pass gathers,

@@ -271,14 +271,18 @@ k_fused(Args a, uint32_t* ticket) {
 // most kBatchOps ops), then saves its state, so launches can be repeated,
 // and LPs added between them, until every LP is done.
 //
-// Every op is computed as kVWaves virtual waves: virtual wave v takes
-// slices v, v + kVWaves, ... and reduces its lanes' partials on its own;
-// the kVWaves results are combined in order.  A workgroup of WPB physical
-// waves (1, 2 or 4) runs virtual waves w, w + WPB, ... one after the
-// other, so the sums, and the LP's whole trajectory, are the same for
-// every WPB: the workgroup size is a pure speed choice.
+// Every op is computed as V virtual waves (Args::vwaves: 8 for the LPs of
+// the largest size class, whose operators have more than 64 slices, 4 for
+// the others; a function of the LP alone): virtual wave v takes slices v,
+// v + V, ... and reduces its lanes' partials on its own; the V results are
+// combined in order.  A workgroup of WPB physical waves (1, 2, 4, or 8 for
+// V = 8) runs virtual waves w, w + WPB, ... one after the other, so the
+// sums, and the LP's whole trajectory, are the same for every WPB: the
+// workgroup size is a pure speed choice.
 constexpr uint32_t kBatchOps = 1u << 22;
-constexpr uint32_t kVWaves = 4;
+constexpr uint32_t kVWaves = 4;     // virtual waves of most LPs
+constexpr uint32_t kMaxVWaves = 8;  // and of the largest size class
+constexpr uint32_t kBigSlices = 64; // more slices than this: kMaxVWaves
 
 struct BlockG {
   uint32_t lane, gwave, nwaves;
@@ -376,10 +380,10 @@ __attribute__((amdgpu_waves_per_eu(3)))
 k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
         uint64_t budget, uint32_t* done_count, uint64_t* t_done,
         uint64_t* bytes, uint32_t* flags, uint64_t* ticks) {
-  static_assert(kVWaves % WPB == 0, "virtual waves per physical wave");
+  static_assert(kMaxVWaves % WPB == 0, "virtual waves per physical wave");
   __shared__ Args la;
   __shared__ State ls;
-  __shared__ double red[kVWaves][kRes];
+  __shared__ double red[kMaxVWaves][kRes];
   __shared__ double res[kRes];
   const uint32_t lp = slots[blockIdx.x];
   const uint32_t tid = threadIdx.x;
@@ -436,14 +440,15 @@ k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
   for (uint32_t guard = 0; guard < kBatchOps; guard++) {
     if (ls.op != OP_NONE) {
       const uint32_t mm = ls.op_maxmask;
-      for (uint32_t v = tid >> 6; v < kVWaves; v += WPB) {
+      const uint32_t nv = la.vwaves;
+      for (uint32_t v = tid >> 6; v < nv; v += WPB) {
         // Reload the LDS records in each pass rather than holding them in
         // registers across passes (which would spill).
         asm volatile("" ::: "memory");
         BlockG g;
         g.lane = tid & 63;
         g.gwave = v;
-        g.nwaves = kVWaves;
+        g.nwaves = nv;
         Ops<BlockG, VI, VC, MT> ops(g, la, ls);
         ops.template run<true>();
 #pragma unroll
@@ -456,7 +461,7 @@ k_batch(const Args* __restrict__ args, State* states, const uint32_t* slots,
       if (tid < kRes) {
         const bool mx = (mm >> tid) & 1;
         double acc = red[0][tid];
-        for (uint32_t w = 1; w < kVWaves; w++)
+        for (uint32_t w = 1; w < nv; w++)
           acc = combine(acc, red[w][tid], mx);
         res[tid] = acc;
       }
@@ -622,6 +627,9 @@ Args make_args(acd_dev_lp* lp, int key) {
   a.ev = lp->ev;
   a.ev_cap = kEventCap;
   a.nnz = lp->orig->nnz;
+  a.vwaves = std::max(lp->perm->a.nslices, lp->perm->at.nslices) > kBigSlices
+                 ? kMaxVWaves
+                 : kVWaves;
   a.si = kSizes[key][0];
   a.sc = kSizes[key][1];
   a.sv = kSizes[key][2];
@@ -993,6 +1001,8 @@ void launch_batch(int key, int wpb, uint32_t nblocks, const uint32_t* slots,
     launch_batch_w<1>(key, nblocks, slots, budget, b, st, shmem);
   else if (wpb == 2)
     launch_batch_w<2>(key, nblocks, slots, budget, b, st, shmem);
+  else if (wpb == 8)
+    launch_batch_w<8>(key, nblocks, slots, budget, b, st, shmem);
   else
     launch_batch_w<4>(key, nblocks, slots, budget, b, st, shmem);
 }
@@ -1005,20 +1015,24 @@ void launch_batch(int key, int wpb, uint32_t nblocks, const uint32_t* slots,
 // three) launches of a round run on their own streams so that they
 // overlap.
 constexpr int kClasses = ACD_DEV_BATCH_CLASSES;
-constexpr int kWpbChoices = 3;
-constexpr int kWpbOf[kWpbChoices] = {1, 2, 4};
+constexpr int kWpbChoices = 4;
+constexpr int kWpbOf[kWpbChoices] = {1, 2, 4, 8};
 
 // The tuner's candidate configurations: workgroup size (index into
 // kWpbOf) per size class.  Fewer waves per LP cost an LP more time per
 // op (its slices run in more passes) but leave more of the machine to
-// the others; 4 waves is the latency-optimal size.
+// the others; 4 waves is the latency-optimal size of the 4-virtual-wave
+// classes.  The largest class (8 virtual waves) runs 8 waves in every
+// candidate: these LPs need the most iterations and set the batch's
+// makespan (measured on 4,096 LPs: 8 virtual waves for them, with twice
+// the LDS and launched first, 765 against 619 solves/s).
 constexpr int kConfigs = 5;
 constexpr uint8_t kConfig[kConfigs][kClasses] = {
-    {2, 2, 2, 2, 2, 2, 2, 2},  // 4 waves
-    {1, 1, 1, 2, 2, 2, 2, 2},  // 2 waves up to 4 slices, 4 above
-    {1, 1, 1, 1, 2, 2, 2, 2},  // 2 waves up to 8 slices, 4 above
-    {1, 1, 1, 1, 1, 2, 2, 2},  // 2 waves up to 16 slices, 4 above
-    {0, 0, 1, 1, 2, 2, 2, 2},  // 1 wave up to 2 slices, 2 up to 8, then 4
+    {2, 2, 2, 2, 2, 2, 2, 3},  // 4 waves (8 for the largest class)
+    {1, 1, 1, 2, 2, 2, 2, 3},  // 2 waves up to 4 slices, 4 above
+    {1, 1, 1, 1, 2, 2, 2, 3},  // 2 waves up to 8 slices, 4 above
+    {1, 1, 1, 1, 1, 2, 2, 3},  // 2 waves up to 16 slices, 4 above
+    {0, 0, 1, 1, 2, 2, 2, 3},  // 1 wave up to 2 slices, 2 up to 8, then 4
 };
 
 int size_class(const acd_dev_lp* lp) {
@@ -1047,8 +1061,10 @@ double now_s() {
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+// 8 physical waves only for LPs of 8 virtual waves (the largest class).
 int lp_choice(int cls, int forced, int cfg) {
-  return forced >= 0 ? forced : kConfig[cfg][cls];
+  const int c = forced >= 0 ? forced : kConfig[cfg][cls];
+  return c == kWpbChoices - 1 && cls < kClasses - 1 ? kWpbChoices - 2 : c;
 }
 
 struct GroupStreams {
@@ -1120,7 +1136,10 @@ int run_round(int key, Group* grp, uint32_t launches, uint64_t budget,
   }
   if (start && hipEventRecord(start, S()) != hipSuccess) return -1;
   if (gs && hipEventRecord(gs->fork, S()) != hipSuccess) return -1;
-  for (int i = 0; i < kWpbChoices; i++) {
+  // Widest workgroups first: their LPs (the largest) start at once
+  // instead of waiting for slots the other groups' workgroups fill
+  // (measured: 765 against 665 solves/s on 4,096 LPs).
+  for (int i = kWpbChoices - 1; i >= 0; i--) {
     Group& g = grp[i];
     if (!g.n) continue;
     hipStream_t st = gs ? gs->s[i] : S();
@@ -1450,8 +1469,15 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
       h_args[i].prof = nullptr;
       h_args[i].partials = nullptr;
       if (lds_on && q->mode == ACD_MODE_ALM)
-        h_args[i].resident = lds_plan(lps[i]->m, lps[i]->n, h_args[i].si,
-                                      lds_vector_budget(), &lds_bytes[i]);
+        // An 8-wave workgroup takes two thirds of a CU's wave slots; it
+        // gets twice the LDS of a 4-wave one (the CU then holds one of
+        // each), enough for the largest LPs' candidate-point pair.
+        h_args[i].resident = lds_plan(
+            lps[i]->m, lps[i]->n, h_args[i].si,
+            h_args[i].vwaves == kMaxVWaves
+                ? lds_vector_budget() + kLdsPerWorkgroup
+                : lds_vector_budget(),
+            &lds_bytes[i]);
       acd_solve_request qi = *q;
       if (!logged) qi.log = 0;
       init_state(&h_st[i], lps[i], &qi);

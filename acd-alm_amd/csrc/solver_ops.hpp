@@ -243,6 +243,7 @@ struct Args {
   uint64_t nnz;                 // nonzeros of A
   uint32_t si, sc, sv;          // bytes per iterate / constant / value
   uint32_t resident;            // batched kernel: vector pairs held in LDS
+  uint32_t vwaves, pad_v;       // batched kernel: the LP's virtual waves
 };
 static_assert(sizeof(Args) % 8 == 0, "Args is copied as 64-bit words");
 

@@ -1,5 +1,6 @@
 """Batched solves (alm_solve_batch): many independent LPs, one workgroup
-each, in repeated launches of 32 inner iterations.
+each, in repeated launches (32 inner iterations while the tuner compares
+workgroup sizes, then to completion).
 
 Each LP's arithmetic depends only on that LP (its own workgroup, its own
 fixed-order reductions), so a result is bit-identical alone and in any
@@ -166,15 +167,19 @@ def test_writing_a_log_does_not_change_the_result(gpu):
 
 
 def test_result_does_not_depend_on_the_workgroup_size(gpu, monkeypatch):
-    """Every op is reduced as 4 virtual waves, so workgroups of 1, 2 or 4
-    wavefronts give bit-identical results."""
+    """Every op is reduced as a fixed number of virtual waves per LP (4, or
+    8 for the largest size class: operators of more than 64 slices), so
+    workgroups of 1, 2, 4 (or 8) wavefronts give bit-identical results."""
     lps = I.batch(10, seed=13, mmin=20, mmax=600)
+    lps.append(I.random_lp(1800, 4800, 5, 17))  # 75 slices: 8 virtual waves
     outs = []
-    for w in (1, 2, 4):
+    for w in (1, 2, 4, 8):
         monkeypatch.setenv("ACD_BATCH_WAVES", str(w))
         print(f"[batch] waves={w}", flush=True)
         out, _, info = batch(gpu, lps, 1)
-        assert set(info.waves_per_lp) <= {0, w}
+        # 8 physical waves only where there are 8 virtual ones
+        assert set(info.waves_per_lp) <= ({0, 4, 8} if w == 8 else {0, w})
+        assert info.waves_per_lp[7] == w
         outs.append(out)
     for o in outs[1:]:
         for i, (r0, r) in enumerate(zip(outs[0], o)):
@@ -190,7 +195,7 @@ def test_tuned_batch_gives_the_forced_results(gpu, monkeypatch):
     print("[batch] tuned 200", flush=True)
     tuned, _, info = batch(gpu, lps, 1)
     assert any(v in (1, 2, 4) for v in info.waves_per_lp)
-    assert all(v in (0, 1, 2, 4) for v in info.waves_per_lp)
+    assert all(v in (0, 1, 2, 4, 8) for v in info.waves_per_lp)
     monkeypatch.setenv("ACD_BATCH_WAVES", "4")
     sample = [0, 7, 50, 123, 199]
     fixed, _, _ = batch(gpu, [lps[i] for i in sample], 1)

@@ -1012,7 +1012,7 @@ void launch_batch(int key, int wpb, uint32_t nblocks, const uint32_t* slots,
 // (class k: at most 2^k slices of 64 rows; the last class holds the rest),
 // and each class runs with the workgroup size of the current
 // configuration.  LPs of one workgroup size share a launch; the (at most
-// three) launches of a round run on their own streams so that they
+// four) launches of a round run on their own streams so that they
 // overlap.
 constexpr int kClasses = ACD_DEV_BATCH_CLASSES;
 constexpr int kWpbChoices = 4;
@@ -1165,7 +1165,7 @@ void drop_graphs(Group* grp) {
     }
 }
 
-// Split `live` (in launch order) into the three workgroup-size groups by
+// Split `live` (in launch order) into the four workgroup-size groups by
 // choice[i] (index into kWpbOf) and upload the lists, group 0 first, into
 // d_slots.  Order inside a group is kept.
 int upload_groups(const uint32_t* live, uint32_t n, const uint8_t* choice,
@@ -1446,7 +1446,7 @@ int acd_dev_solve_batch(struct acd_dev_lp* const* lps, size_t count,
     acd_dev_record_error(-1, "batch allocation");
     rc = -1;
   }
-  // Workgroup size per LP: forced (ACD_BATCH_WAVES=1|2|4), or the
+  // Workgroup size per LP: forced (ACD_BATCH_WAVES=1|2|4|8), or the
   // configuration the tuner picks on this batch's own rounds.
   const int forced = forced_choice();
   const bool tune = forced < 0 && count >= kTuneMinLps;

@@ -308,6 +308,7 @@ def run_batch(args, lib):
     L.acd_synchronize()
     dev_ms = ker_ms = alg = walls = ops_ms = ctl_ms = 0.0
     launches = inner = 0
+    inner_max = (0, 0, 0)
     dones = []
     for k in range(args.steps):
         out, done, info, wall = step()
@@ -318,6 +319,9 @@ def run_batch(args, lib):
         alg += info.alg_bytes
         launches += int(info.launches)
         inner += int(info.inner_iterations)
+        big = int(np.argmax(info.inner))
+        if int(info.inner[big]) > inner_max[0]:
+            inner_max = (int(info.inner[big]), lps[big].nrows, lps[big].ncols)
         walls += wall
         dones.append(done)
         wpl = list(info.waves_per_lp)
@@ -336,13 +340,19 @@ def run_batch(args, lib):
         "kkt_max": float(max(kk)), "kkt_ok": bool(max(kk) <= KKT_TOL),
         "wall_solves_per_s": count * args.steps / walls,
         "inner_iterations_per_lp": inner / args.steps / count,
+        "inner_iterations_max": {"iterations": inner_max[0],
+                                 "m": inner_max[1], "n": inner_max[2]},
         "waves_per_lp_by_class": wpl,
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": "k_batch (batched solver kernel, one workgroup per "
-                      "LP, 32 inner iterations per launch)",
+                      "LP; 32 inner iterations per launch while tuning, "
+                      "then each LP to completion)",
+            "timing": "HIP events around each round; the workgroup-size "
+                      "groups of a round run concurrently on their own "
+                      "streams, so avg_us is round time per group launch",
             "launches": launches,
             "avg_us": round(ker_ms * 1e3 / max(launches, 1), 1),
             "alg_bytes_per_launch": round(alg / max(launches, 1)),

@@ -60,6 +60,9 @@ struct acd_sell {
         uint16_t * val16;      /* device [nent] bf16 bits, or NULL */
         uint32_t * colsgn;     /* device [nent] column | sign bits when every
                                   value is -1, 0 or +1 (value-free), or NULL */
+        uint16_t * col16;      /* device [nent] columns as 16 bits when the
+                                  operator has at most 65,536 columns, or
+                                  NULL */
         uint32_t * rowlen;     /* device [nrows]: entries per row */
         int32_t tuned;         /* SpMV variant + 1 picked by the autotuner,
                                   0 = library default */
@@ -97,6 +100,11 @@ int acd_dev_sell_make_bf16(struct acd_sell *);
  * Returns 1 when made, 0 when some value is not -1, 0 or +1 or some
  * column does not fit in 30 bits (no copy), -1 on error. */
 int acd_dev_sell_make_sign(struct acd_sell *);
+/* 16-bit copy of the column indices (col16), made when the operator has
+ * at most 65,536 columns (padding entries repeat a column of their row, or
+ * 0, so every stored index is below ncols).  Returns 1 when made, 0 when
+ * the operator is too wide (no copy), -1 on error. */
+int acd_dev_sell_make_col16(struct acd_sell *);
 
 /* ---- vector kernels (dev_spmv.hip) ---- */
 /* y = op x on device pointers (op = A' when transpose). */

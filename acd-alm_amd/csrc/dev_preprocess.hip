@@ -204,6 +204,11 @@ __global__ void k_to_sign(const double* val, const uint32_t* col,
   dst[i] = w;
 }
 
+__global__ void k_to_u16(const uint32_t* src, uint16_t* dst, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (uint16_t)src[i];
+}
+
 struct Scratch {
   void* p = nullptr;
   size_t bytes = 0;
@@ -361,6 +366,7 @@ void free_sell(struct acd_sell* op) {
   hipFree(op->val32);
   hipFree(op->val16);
   hipFree(op->colsgn);
+  hipFree(op->col16);
   hipFree(op->rowlen);
   memset(op, 0, sizeof *op);
 }
@@ -581,6 +587,27 @@ int acd_dev_sell_make_sign(struct acd_sell* op) {
     return 0;
   }
   op->colsgn = w;
+  return 1;
+}
+
+int acd_dev_sell_make_col16(struct acd_sell* op) {
+  if (op->col16) return 1;
+  if (op->ncols > 65536u) return 0;
+  uint16_t* c = nullptr;
+  if (hipMalloc((void**)&c, sizeof(uint16_t) * (op->nent ? op->nent : 1)) !=
+      hipSuccess) {
+    acd_dev_record_error(-1, "hipMalloc 16-bit columns");
+    return -1;
+  }
+  if (op->nent)
+    k_to_u16<<<blocks_for(op->nent, 256), 256, 0, S()>>>(op->col, c,
+                                                         op->nent);
+  if (hipGetLastError() != hipSuccess) {
+    hipFree(c);
+    acd_dev_record_error(-1, "16-bit columns");
+    return -1;
+  }
+  op->col16 = c;
   return 1;
 }
 

@@ -129,6 +129,7 @@ struct Sell {
   const float* val32;      // FP32 copy (mixed precision) or NULL
   const uint16_t* val16;   // BF16 copy when every value is exact, or NULL
   const uint32_t* colsgn;  // value-free entries of a 0/+-1 operator, or NULL
+  const uint16_t* col16;   // columns as 16 bits (at most 65,536), or NULL
   uint32_t nrows, nslices;
 };
 
@@ -136,6 +137,10 @@ struct Sell {
 // or +1 and travels in the column word (bit 31: -1, bit 30: 0), so an
 // entry costs 4 bytes instead of 6 (BF16) or 8 (FP32).
 struct SgnT {};
+
+// Matrix-value type of FP32 values with 16-bit column indices (operators
+// of at most 65,536 columns): 6 bytes per entry instead of 8.
+struct F32C16 {};
 
 // Column of entry e as stored for precision MT.
 template <class MT>
@@ -145,6 +150,10 @@ __device__ __forceinline__ uint32_t mcol(const Sell& M, uint64_t e) {
 template <>
 __device__ __forceinline__ uint32_t mcol<SgnT>(const Sell& M, uint64_t e) {
   return M.colsgn[e] & 0x3fffffffu;
+}
+template <>
+__device__ __forceinline__ uint32_t mcol<F32C16>(const Sell& M, uint64_t e) {
+  return M.col16[e];
 }
 
 // Value of entry e as stored for precision MT (double, float, or bf16
@@ -164,10 +173,129 @@ __device__ __forceinline__ double mval<uint16_t>(const Sell& M, uint64_t e) {
   return (double)__uint_as_float((uint32_t)M.val16[e] << 16);
 }
 template <>
+__device__ __forceinline__ double mval<F32C16>(const Sell& M, uint64_t e) {
+  return (double)M.val32[e];
+}
+template <>
 __device__ __forceinline__ double mval<SgnT>(const Sell& M, uint64_t e) {
   const uint32_t w = M.colsgn[e];
   return (w & 0x40000000u) ? 0.0 : ((w & 0x80000000u) ? -1.0 : 1.0);
 }
+
+// One stored entry as loaded, decoded only where it is used.  The row
+// loop issues the next chunk's entry loads while the current chunk's
+// gathers are in flight and waits for both at one point (pin()); a
+// conversion at the load (FP32 or BF16 to FP64, the value-free word's bit
+// tests) would make the compiler wait for the loads right there, before
+// the gathers are issued, serialising the two memory round trips.
+template <class T>
+__device__ __forceinline__ T gload(const T* p, uint64_t e) {
+  return ((const __attribute__((address_space(1))) T*)p)[e];
+}
+// A pointer that is the same in every lane (read from a record all lanes
+// share), moved to scalar registers.
+template <class T>
+__device__ __forceinline__ const T* wave_ptr(const T* p) {
+  const uint64_t v = (uint64_t)p;
+  const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint64_t hi =
+      (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (const T*)((hi << 32) | lo);
+}
+
+// Base pointers of the entry streams that precision MT reads, kept in
+// scalar registers for a whole product (the empty asm statement makes
+// them opaque, so the compiler cannot re-read them from the LDS-resident
+// argument record between loads).
+template <class MT>
+struct EntBase {
+  const uint32_t* col;
+  const MT* val;
+  __device__ __forceinline__ explicit EntBase(const Sell& M);
+};
+template <>
+__device__ __forceinline__ EntBase<double>::EntBase(const Sell& M)
+    : col(wave_ptr(M.col)), val(wave_ptr(M.val)) {
+  asm volatile("" : "+s"(col), "+s"(val));
+}
+template <>
+__device__ __forceinline__ EntBase<float>::EntBase(const Sell& M)
+    : col(wave_ptr(M.col)), val(wave_ptr(M.val32)) {
+  asm volatile("" : "+s"(col), "+s"(val));
+}
+template <>
+__device__ __forceinline__ EntBase<uint16_t>::EntBase(const Sell& M)
+    : col(wave_ptr(M.col)), val(wave_ptr(M.val16)) {
+  asm volatile("" : "+s"(col), "+s"(val));
+}
+template <>
+struct EntBase<SgnT> {
+  const uint32_t* w;
+  __device__ __forceinline__ explicit EntBase(const Sell& M)
+      : w(wave_ptr(M.colsgn)) {
+    asm volatile("" : "+s"(w));
+  }
+};
+template <>
+struct EntBase<F32C16> {
+  const uint16_t* col;
+  const float* val;
+  __device__ __forceinline__ explicit EntBase(const Sell& M)
+      : col(wave_ptr(M.col16)), val(wave_ptr(M.val32)) {
+    asm volatile("" : "+s"(col), "+s"(val));
+  }
+};
+
+template <class MT>
+struct Ent {
+  uint32_t c;
+  MT v;
+  __device__ __forceinline__ void load(const EntBase<MT>& B, uint64_t e) {
+    c = gload(B.col, e);
+    v = gload(B.val, e);
+  }
+  __device__ __forceinline__ uint32_t col() const { return c; }
+  __device__ __forceinline__ double val() const { return (double)v; }
+  __device__ __forceinline__ void pin() { asm volatile("" : "+v"(c), "+v"(v)); }
+};
+template <>
+struct Ent<uint16_t> {  // BF16 bits, zero-extended
+  uint32_t c, v;
+  __device__ __forceinline__ void load(const EntBase<uint16_t>& B,
+                                       uint64_t e) {
+    c = gload(B.col, e);
+    v = gload(B.val, e);
+  }
+  __device__ __forceinline__ uint32_t col() const { return c; }
+  __device__ __forceinline__ double val() const {
+    return (double)__uint_as_float(v << 16);
+  }
+  __device__ __forceinline__ void pin() { asm volatile("" : "+v"(c), "+v"(v)); }
+};
+template <>
+struct Ent<SgnT> {
+  uint32_t w;
+  __device__ __forceinline__ void load(const EntBase<SgnT>& B, uint64_t e) {
+    w = gload(B.w, e);
+  }
+  __device__ __forceinline__ uint32_t col() const { return w & 0x3fffffffu; }
+  __device__ __forceinline__ double val() const {
+    return (w & 0x40000000u) ? 0.0 : ((w & 0x80000000u) ? -1.0 : 1.0);
+  }
+  __device__ __forceinline__ void pin() { asm volatile("" : "+v"(w)); }
+};
+template <>
+struct Ent<F32C16> {
+  uint32_t c;  // zero-extended 16-bit column
+  float v;
+  __device__ __forceinline__ void load(const EntBase<F32C16>& B, uint64_t e) {
+    c = gload(B.col, e);
+    v = gload(B.val, e);
+  }
+  __device__ __forceinline__ uint32_t col() const { return c; }
+  __device__ __forceinline__ double val() const { return (double)v; }
+  __device__ __forceinline__ void pin() { asm volatile("" : "+v"(c), "+v"(v)); }
+};
 
 struct State {
   // control
@@ -243,7 +371,8 @@ struct Args {
   uint64_t nnz;                 // nonzeros of A
   uint32_t si, sc, sv;          // bytes per iterate / constant / value
   uint32_t resident;            // batched kernel: vector pairs held in LDS
-  uint32_t vwaves, pad_v;       // batched kernel: the LP's virtual waves
+  uint32_t vwaves;              // batched kernel: the LP's virtual waves
+  uint32_t sx;                  // bytes per stored column index
 };
 static_assert(sizeof(Args) % 8 == 0, "Args is copied as 64-bit words");
 
@@ -343,7 +472,10 @@ struct Ops {
       uint64_t e = base + g.lane;
       double a0 = 0.0, a1 = 0.0;
       uint32_t k = 0;
-      if (len >= 4) {
+      if (len >= 4 && !G::kPipelinedRows) {
+        // Per chunk of 4 entries: the next chunk's entries are loaded
+        // (and waited for), then the gathers of its products are issued,
+        // then the adds in entry order.
         uint32_t c0 = mcol<MT2>(M, e), c1 = mcol<MT2>(M, e + 64),
                  c2 = mcol<MT2>(M, e + 128), c3 = mcol<MT2>(M, e + 192);
         double v0 = mval<MT2>(M, e), v1 = mval<MT2>(M, e + 64),
@@ -380,6 +512,50 @@ struct Ops {
           if (!more) break;
           c0 = d0; c1 = d1; c2 = d2; c3 = d3;
           v0 = w0; v1 = w1; v2 = w2; v3 = w3;
+        }
+      } else if (len >= 4) {
+        // Per chunk of 4 entries: the gathers of its products are issued,
+        // then the next chunk's entry loads, then one wait for both (pin:
+        // the gathered and the loaded registers pass through an empty asm
+        // statement, so neither load can move past it and nothing is
+        // computed from them before it), then the adds in entry order.
+        const EntBase<MT2> B(M);
+        Ent<MT2> cur[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) cur[q].load(B, e + 64 * q);
+        for (;;) {
+          const bool more = k + 8 <= len;
+          X g0[4], g1[4];
+#pragma unroll
+          for (int q = 0; q < 4; q++) g0[q] = x0[cur[q].col()];
+          if (NRHS == 2) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) g1[q] = x1[cur[q].col()];
+          }
+          Ent<MT2> nxt[4] = {};
+          if (more) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) nxt[q].load(B, e + 256 + 64 * q);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            asm volatile("" : "+v"(g0[q]));
+            if (NRHS == 2) asm volatile("" : "+v"(g1[q]));
+            nxt[q].pin();
+          }
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            a0 = a0 + cur[q].val() * (double)g0[q];
+          if (NRHS == 2) {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+              a1 = a1 + cur[q].val() * (double)g1[q];
+          }
+          k += 4;
+          e += 256;
+          if (!more) break;
+#pragma unroll
+          for (int q = 0; q < 4; q++) cur[q] = nxt[q];
         }
       }
       // The last len % 4 entries as one batch: their loads and gathers
@@ -995,7 +1171,7 @@ struct Ops {
 // the achieved fraction of the bandwidth roofline exposes.
 __device__ inline double alg_bytes(int32_t op, uint32_t fl, const Args& a) {
   const double m = a.m, n = a.n, si = a.si, sc = a.sc;
-  const double mat = (double)a.nnz * (a.sv + 4.0);
+  const double mat = (double)a.nnz * (a.sv + a.sx);
   const double offA = 8.0 * (a.A.nslices + 1), offT = 8.0 * (a.AT.nslices + 1);
   const double spA = mat + offA, spT = mat + offT;  // A rows / A' rows
   switch (op) {

@@ -515,16 +515,18 @@ struct Ops {
         }
       } else if (len >= 4) {
         // Per chunk of 4 entries: the gathers of its products are issued,
-        // then the next chunk's entry loads, then one wait for both (pin:
+        // then the next chunk's entry loads (after the last full chunk:
+        // the row's last len % 4 entries), then one wait for both (pin:
         // the gathered and the loaded registers pass through an empty asm
         // statement, so neither load can move past it and nothing is
         // computed from them before it), then the adds in entry order.
         const EntBase<MT2> B(M);
-        Ent<MT2> cur[4];
+        Ent<MT2> cur[4], nxt[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) cur[q].load(B, e + 64 * q);
         for (;;) {
           const bool more = k + 8 <= len;
+          const uint32_t nn = more ? 4u : len - k - 4;  // entries in nxt
           X g0[4], g1[4];
 #pragma unroll
           for (int q = 0; q < 4; q++) g0[q] = x0[cur[q].col()];
@@ -532,10 +534,10 @@ struct Ops {
 #pragma unroll
             for (int q = 0; q < 4; q++) g1[q] = x1[cur[q].col()];
           }
-          Ent<MT2> nxt[4] = {};
-          if (more) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) nxt[q].load(B, e + 256 + 64 * q);
+          for (int q = 0; q < 4; q++) {
+            nxt[q] = {};
+            if ((uint32_t)q < nn) nxt[q].load(B, e + 256 + 64 * q);
           }
 #pragma unroll
           for (int q = 0; q < 4; q++) {
@@ -556,6 +558,26 @@ struct Ops {
           if (!more) break;
 #pragma unroll
           for (int q = 0; q < 4; q++) cur[q] = nxt[q];
+        }
+        // The last len % 4 entries, loaded with the last full chunk.
+        const uint32_t tail = len - k;
+        if (tail) {
+          const bool h1 = tail > 1, h2 = tail > 2;
+          const double p0 = (double)x0[nxt[0].col()];
+          const double p1 = h1 ? (double)x0[nxt[1].col()] : 0.0;
+          const double p2 = h2 ? (double)x0[nxt[2].col()] : 0.0;
+          a0 = a0 + nxt[0].val() * p0;
+          if (h1) a0 = a0 + nxt[1].val() * p1;
+          if (h2) a0 = a0 + nxt[2].val() * p2;
+          if (NRHS == 2) {
+            const double q0 = (double)x1[nxt[0].col()];
+            const double q1 = h1 ? (double)x1[nxt[1].col()] : 0.0;
+            const double q2 = h2 ? (double)x1[nxt[2].col()] : 0.0;
+            a1 = a1 + nxt[0].val() * q0;
+            if (h1) a1 = a1 + nxt[1].val() * q1;
+            if (h2) a1 = a1 + nxt[2].val() * q2;
+          }
+          k = len;
         }
       }
       // The last len % 4 entries as one batch: their loads and gathers

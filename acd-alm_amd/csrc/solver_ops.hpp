@@ -513,6 +513,48 @@ struct Ops {
           c0 = d0; c1 = d1; c2 = d2; c3 = d3;
           v0 = w0; v1 = w1; v2 = w2; v3 = w3;
         }
+      } else if (len >= 4 && len < 8) {
+        // One full chunk and a tail of len - 4 entries (every A' row of an
+        // LP with 5 nonzeros per column): all entries are loaded at once,
+        // then all gathers are issued, then the adds in entry order.
+        const EntBase<MT2> B(M);
+        const uint32_t t = len - 4;
+        Ent<MT2> cur[4], tl[3];
+#pragma unroll
+        for (int q = 0; q < 4; q++) cur[q].load(B, e + 64 * q);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          tl[q] = {};
+          if ((uint32_t)q < t) tl[q].load(B, e + 256 + 64 * q);
+        }
+        X g0[7], g1[7];
+#pragma unroll
+        for (int q = 0; q < 4; q++) g0[q] = x0[cur[q].col()];
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+          g0[4 + q] = (uint32_t)q < t ? x0[tl[q].col()] : X(0);
+        if (NRHS == 2) {
+#pragma unroll
+          for (int q = 0; q < 4; q++) g1[q] = x1[cur[q].col()];
+#pragma unroll
+          for (int q = 0; q < 3; q++)
+            g1[4 + q] = (uint32_t)q < t ? x1[tl[q].col()] : X(0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          a0 = a0 + cur[q].val() * (double)g0[q];
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+          if ((uint32_t)q < t) a0 = a0 + tl[q].val() * (double)g0[4 + q];
+        if (NRHS == 2) {
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            a1 = a1 + cur[q].val() * (double)g1[q];
+#pragma unroll
+          for (int q = 0; q < 3; q++)
+            if ((uint32_t)q < t) a1 = a1 + tl[q].val() * (double)g1[4 + q];
+        }
+        k = len;
       } else if (len >= 4) {
         // Per chunk of 4 entries: the gathers of its products are issued,
         // then the next chunk's entry loads (after the last full chunk:

@@ -182,12 +182,9 @@ __device__ __forceinline__ double mval<SgnT>(const Sell& M, uint64_t e) {
   return (w & 0x40000000u) ? 0.0 : ((w & 0x80000000u) ? -1.0 : 1.0);
 }
 
-// One stored entry as loaded, decoded only where it is used.  The row
-// loop issues the next chunk's entry loads while the current chunk's
-// gathers are in flight and waits for both at one point (pin()); a
-// conversion at the load (FP32 or BF16 to FP64, the value-free word's bit
-// tests) would make the compiler wait for the loads right there, before
-// the gathers are issued, serialising the two memory round trips.
+// Element e of a matrix stream, read as global memory (the streams never
+// live in LDS), so the load counts only against the vector-memory counter
+// and a wait for it does not also wait for LDS reads.
 template <class T>
 __device__ __forceinline__ T gload(const T* p, uint64_t e) {
   return ((const __attribute__((address_space(1))) T*)p)[e];
@@ -246,6 +243,12 @@ struct EntBase<F32C16> {
   }
 };
 
+// One stored entry as loaded, decoded only where it is used.  The
+// batched row loop issues the next chunk's entry loads while the current
+// chunk's gathers are in flight and waits for both at one point (pin()); a
+// conversion at the load (FP32 or BF16 to FP64, the value-free word's bit
+// tests) would make the compiler wait for the loads right there, before
+// the gathers are issued, serialising the two memory round trips.
 template <class MT>
 struct Ent {
   uint32_t c;
